@@ -1,0 +1,191 @@
+"""Benchmark of the CCD++ hot path on one MI355X.
+
+Workload: the 480,189 x 17,770 matrix with 99,072,112 training and
+1,408,395 test ratings (seeded synthetic, power-law degrees), k=40,
+lambda=0.05, 5 inner iterations.  One step = one outer iteration: all 40
+rank-one updates with 5 inner sweeps each over the whole matrix, plus the
+residual updates and the device test RMSE.  The default --steps 10 is the
+configuration's T=10 solve, timed from the reference initial state
+(W from initial_col, H = 0) with the data already resident in HBM.
+
+Prints one JSON line (metric, value = nnz*k updates/s, roofline of the
+dominant kernel measured with HIP events inside the timed region,
+cpu_baseline = the OpenMP oracle on a bounded sample of the same workload).
+
+usage: python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ds, W0, k, lam, inner, budget_s=15.0):
+    """OpenMP oracle (the restated cdmf_ref loops, schedule(dynamic,500)) on
+    the same matrix: whole ranks of a steady-state outer iteration (add-back,
+    5 inner sweeps, subtract), as many as fit in `budget_s`."""
+    import numpy as np
+
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    H0 = np.zeros((k, ds.n), np.float32)
+    ranks = 0
+    t0 = time.perf_counter()
+    W, H = W0, H0
+    while ranks < k:
+        W, H = oracle.ccd_ranks(ds, W, H, ranks, ranks + 1, lam, 2, inner, threads=threads)
+        ranks += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": ranks * ds.nnz / dt,
+        "unit": "nnz*k updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "ranks 0..%d of one steady-state outer iteration (add-back, %d inner sweeps, subtract) on the "
+                  "full %dx%d matrix, %.1f s, %d OpenMP threads on %s (each rank copies R's values first)"
+                  % (ranks - 1, inner, ds.m, ds.n, dt, threads, cpu_model()),
+    }
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--small", action="store_true", help="the 2,000 x 1,000 case instead (quick check)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from cdmf_amd import data as D
+    from cdmf_amd import solver as S
+
+    if S.device_count() < 1:
+        sys.exit("bench.py: no HIP device visible")
+
+    if args.small:
+        cfg, spec = D.SMALL_SOLVE, D.SMALL
+        t = time.perf_counter()
+        ds = D.small()
+    else:
+        cfg, spec = D.NETFLIX_SOLVE, D.NETFLIX
+        t = time.perf_counter()
+        ds = D.netflix()
+    log("generated %dx%d nnz=%d test=%d in %.1fs" % (ds.m, ds.n, ds.nnz, ds.nnz_test, time.perf_counter() - t))
+    k, lam, inner = cfg.k, cfg.lam, cfg.inner
+    W0 = D.initial_col(k, ds.m)
+    H0 = np.zeros((k, ds.n), np.float32)
+
+    s = S.Solver(0)
+    t = time.perf_counter()
+    s.upload(ds)
+    log("uploaded in %.1fs" % (time.perf_counter() - t))
+
+    if args.warmup > 0:
+        s.set_factors(W0, H0)
+        t = time.perf_counter()
+        s.run(k, lam, args.warmup, inner, profile=False)
+        log("warmup %d outer iteration(s) in %.2fs" % (args.warmup, time.perf_counter() - t))
+
+    s.set_factors(W0, H0)
+    t0 = time.perf_counter()
+    res = s.run(k, lam, args.steps, inner, profile=True)  # returns after the stream is drained
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    log("timed %d outer iterations in %.3fs; test RMSE %s" % (args.steps, wall, ["%.6f" % r for r in res.rmse]))
+
+    stats = s.kernel_stats()
+    sweep_kinds = ("sweep_f", "sweep_s", "fused_f", "fused_s")
+    dom = max(sweep_kinds, key=lambda kk: stats[kk]["total_ms"])
+    d = stats[dom]
+    avg_ms = d["total_ms"] / max(d["launches"], 1)
+    achieved = d["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+    kernels = {kk: dict(avg_us=1e3 * v["total_ms"] / max(v["launches"], 1), launches=v["launches"],
+                        gbs=(v["bytes_per_launch"] / (v["total_ms"] / max(v["launches"], 1) * 1e-3) / 1e9)
+                        if v["launches"] else 0.0)
+               for kk, v in stats.items() if v["launches"]}
+    traffic = load_traffic()
+    roofline = {
+        "bound": "hbm",
+        "kernel": {"sweep_f": "sweep (CSC, v update)", "sweep_s": "sweep (CSR, u update)",
+                   "fused_f": "fused residual+sweep (CSC)", "fused_s": "fused residual+sweep (CSR)"}[dom],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get(dom, {}).get("hbm_bytes_per_launch") if traffic else None,
+        "algorithmic_bytes_per_launch": d["bytes_per_launch"],
+        "avg_launch_us": round(avg_ms * 1e3, 2),
+    }
+    value = args.steps * k * ds.nnz / wall
+    out = {
+        "metric": "CCD++ nnz*k updates/s",
+        "value": value,
+        "unit": "nnz*k updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic, seeded power-law ratings (%d x %d, %d train / %d test)" % (ds.m, ds.n, ds.nnz, ds.nnz_test),
+        "config": {"workload": "CCD++ rank-one loop, %d x %d, nnz=%d, k=%d, lambda=%g, %d outer x %d inner"
+                               % (ds.m, ds.n, ds.nnz, k, lam, args.steps, inner),
+                   "m": ds.m, "n": ds.n, "nnz": ds.nnz, "nnz_test": ds.nnz_test, "k": k, "lambda": lam,
+                   "outer": args.steps, "inner": inner, "seed": spec["seed"]},
+        "roofline": roofline,
+        "test_rmse": res.rmse,
+        "loop_time_s": s.loop_time(),
+        "kernels": kernels,
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(ds, W0, k, lam, inner)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

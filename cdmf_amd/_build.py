@@ -1,0 +1,94 @@
+"""Build recipes for the native parts of cdmf_amd (run by __graft_entry__.build()).
+
+Outputs (in-tree, git-ignored, shipped with the tree):
+  cdmf_amd/_lib/libcdmf_data.so   host data layer (g++, OpenMP)
+  cdmf_amd/_lib/libcdmf_hip.so    HIP kernels + C ABI for gfx950 (hipcc)
+  cdmf_amd/bin/cdmf               the command-line driver (flags of src/main.cpp)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+INC = os.path.join(ROOT, "include")
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "_lib")
+BIN = os.path.join(HERE, "bin")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -packed-fp32-ops: no v_pk_*_f32 (some forms have misbehaved on this pool).
+# -ffp-contract=off: residual updates round like the reference (mul, then add).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=off", "-munsafe-fp-atomics", "-Wall", "-Wno-unused-result",
+]
+
+HIP_SOURCES = [
+    "kernels/ccd_kernels.hip",
+    "ccd_abi.hip",
+]
+HOST_SOURCES = [
+    "host/tools.cpp",
+    "host/cdmf_hip.cpp",
+    "host/main.cpp",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_data() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcdmf_data.so")
+    _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall", "-I" + INC,
+          os.path.join(CSRC, "data", "data.cpp"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_hip() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcdmf_hip.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + INC, "-I" + os.path.join(CSRC, "kernels"), *srcs,
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_cli() -> str:
+    os.makedirs(BIN, exist_ok=True)
+    out = os.path.join(BIN, "cdmf")
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I" + INC, "-I" + os.path.join(CSRC, "host"), *srcs,
+          "-L" + LIB, "-Wl,-rpath,$ORIGIN/../_lib", "-lcdmf_hip", "-lcdmf_data", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle() -> None:
+    if shutil.which("make") is None:
+        raise RuntimeError("make not found")
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all() -> None:
+    build_data()
+    build_hip()
+    build_cli()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "data": build_data, "hip": build_hip, "cli": build_cli, "oracle": build_oracle}[w]()

@@ -1,0 +1,689 @@
+// C ABI of the gfx950 CCD++ solver (include/cdmf_hip.h).
+//
+// Device state of one context: both compressed copies of R (pointer, index,
+// residual value and a pristine copy of the values), the factors W [k][m]
+// and H [k][n] (rank-major, as MatData [t][i] in src/pmf_util.h:7-8), the
+// working vectors u, v, the test set and, per layout, the wave work plan of
+// the binned sweeps.
+//
+// The outer loop follows cdmf_ref.cpp:69-137 step for step, with one
+// reordering that leaves every rounded value unchanged: the subtraction of
+// rank t-1's final u v^T (cdmf_ref.cpp:120-121) is deferred and done in the
+// same pass as the add-back of rank t (cdmf_ref.cpp:86-89) and the first
+// sweep of rank t.  Per element the arithmetic is the reference's,
+// (val - u_{t-1} v_{t-1}) + w_t h_t, each product and sum rounded to FP32.
+// After the last rank the pending subtraction is applied on its own, so the
+// residual left on the device equals the one the reference leaves in R.
+#include "cdmf_hip.h"
+#include "ccd_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct DevLayout {
+    int64_t rows = 0;   // rows of this layout (n for CSC, m for CSR)
+    int64_t gdim = 0;   // length of the gathered vector (m for CSC, n for CSR)
+    uint64_t nnz = 0;
+    uint32_t* ptr = nullptr;
+    void* idx = nullptr;
+    bool idx16 = false;
+    float* val = nullptr;   // residual
+    float* val0 = nullptr;  // R itself, to reset the residual
+    uint4* units = nullptr;
+    uint32_t nunits = 0;
+    uint32_t* rowlist = nullptr;
+    uint32_t* split_rows = nullptr;
+    uint32_t* split_beg = nullptr;
+    uint32_t nsplit = 0;
+    float2* partials = nullptr;
+    uint32_t npieces = 0;
+};
+
+struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ccd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool uploaded = false, factors = false;
+    int64_t m = 0, n = 0;
+    uint64_t nnz = 0;
+    DevLayout csr, csc;
+    uint64_t nnz_test = 0;
+    uint32_t* trow = nullptr;
+    uint32_t* tcol = nullptr;
+    float* tval = nullptr;
+    uint32_t k = 0;
+    float* W = nullptr;
+    float* H = nullptr;
+    float* u = nullptr;
+    float* v = nullptr;
+    double* rmse_bsum = nullptr;
+    double* rmse_dev = nullptr;  // one slot per outer iteration
+    int32_t rmse_cap = 0;
+    std::vector<double> iter_rmse;
+    ccd_plan_params plan{4, 8192, 4096, 4096};
+    // profiling
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    double kind_ms[CCD_K_COUNT] = {};
+    uint64_t kind_n[CCD_K_COUNT] = {};
+    double kind_bytes[CCD_K_COUNT] = {};
+    double loop_s = 0.0;
+};
+
+namespace {
+
+int set_err(ccd_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return set_err((ctx), CCD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+void free_layout(DevLayout& L) {
+    dfree(L.ptr);
+    dfree(L.idx);
+    dfree(L.val);
+    dfree(L.val0);
+    dfree(L.units);
+    dfree(L.rowlist);
+    dfree(L.split_rows);
+    dfree(L.split_beg);
+    dfree(L.partials);
+    L = DevLayout{};
+}
+
+// ----------------------------------------------------------- planning ---
+struct HostPlan {
+    std::vector<uint4> units;
+    std::vector<uint32_t> rowlist;
+    std::vector<uint32_t> split_rows, split_beg;
+};
+
+inline int group_log2(uint32_t d, uint32_t epl) {
+    uint32_t want = (d + epl - 1) / (epl ? epl : 1);
+    int lg = 0;
+    while ((1u << lg) < want && lg < 6) ++lg;
+    return lg;
+}
+
+HostPlan make_plan(const uint32_t* ptr, int64_t rows, const ccd_plan_params& pp) {
+    HostPlan P;
+    std::vector<uint32_t> order, split;
+    order.reserve(rows);
+    for (int64_t r = 0; r < rows; ++r) {
+        const uint32_t d = ptr[r + 1] - ptr[r];
+        (d > pp.split_threshold ? split : order).push_back((uint32_t)r);
+    }
+    auto deg = [&](uint32_t r) { return ptr[r + 1] - ptr[r]; };
+    std::stable_sort(split.begin(), split.end(), [&](uint32_t a, uint32_t b) { return deg(a) > deg(b); });
+    uint32_t slot = 0;
+    for (uint32_t r : split) {
+        P.split_rows.push_back(r);
+        P.split_beg.push_back(slot);
+        for (uint32_t s = ptr[r]; s < ptr[r + 1]; s += pp.piece) {
+            const uint32_t e = std::min<uint64_t>((uint64_t)s + pp.piece, ptr[r + 1]);
+            P.units.push_back(make_uint4(ccd::UNIT_SPLIT | (slot << 8), r, s, e));
+            ++slot;
+        }
+    }
+    P.split_beg.push_back(slot);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return deg(a) > deg(b); });
+    P.rowlist = order;
+    const size_t N = order.size();
+    const uint32_t max_rows = 4096;
+    size_t i = 0;
+    while (i < N) {
+        const int lg = group_log2(deg(order[i]), pp.elems_per_lane);
+        const uint32_t rpp = 64u >> lg;
+        size_t j = i;
+        uint64_t elems = 0;
+        while (j < N && group_log2(deg(order[j]), pp.elems_per_lane) == lg) {
+            const size_t cnt = j - i;
+            if (cnt % rpp == 0 && (elems >= pp.unit_elems || cnt >= max_rows)) break;
+            elems += deg(order[j]);
+            ++j;
+        }
+        P.units.push_back(make_uint4(ccd::UNIT_GROUP | ((uint32_t)lg << 8), (uint32_t)i, (uint32_t)j, 0u));
+        i = j;
+    }
+    return P;
+}
+
+// ----------------------------------------------------------- profiling --
+hipEvent_t take_event(ccd_ctx* c) {
+    if (c->pool_used == c->pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->pool.push_back(e);
+    }
+    return c->pool[c->pool_used++];
+}
+
+struct Timed {
+    ccd_ctx* c;
+    int kind;
+    hipEvent_t a = nullptr;
+    Timed(ccd_ctx* ctx, int k) : c(ctx), kind(k) {
+        if (c->prof) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->prof && a) {
+            hipEvent_t b = take_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->recs.push_back({kind, a, b});
+            }
+        }
+    }
+};
+
+// Algorithmic HBM bytes of one pass over layout L: the index and value
+// streams (values read, and written back when the residual is updated), the
+// row pointer and the row list, one read of every distinct gathered vector
+// and per-row vector, and the output vector.
+double layout_bytes(const DevLayout& L, bool residual_write, int gathered, int per_row, bool writes_y) {
+    const double ib = L.idx16 ? 2.0 : 4.0;
+    double b = (double)L.nnz * (ib + 4.0 + (residual_write ? 4.0 : 0.0));
+    b += 4.0 * (double)(L.rows + 1) + 4.0 * (double)L.rows;
+    b += 4.0 * (double)L.gdim * gathered + 4.0 * (double)L.rows * per_row;
+    if (writes_y) b += 4.0 * (double)L.rows;
+    return b;
+}
+
+struct RunState {
+    ccd_ctx* c;
+    const ccd_params* p;
+};
+
+int do_pass(ccd_ctx* c, DevLayout& L, const float* x, float* y, const float* xs, const float* ys,
+            const float* xa, const float* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
+    ccd::SweepArgs a{L.ptr, L.idx, L.val, x, y, xs, ys, xa, ya, L.units, L.rowlist, L.partials, L.nunits, lambda};
+    {
+        Timed tm(c, kind);
+        HIP_TRY(c, ccd::launch_sweep(a, L.idx16, sub, add, sweep, c->stream));
+    }
+    if (sweep && L.nsplit) {
+        Timed tm(c, CCD_K_FINALIZE);
+        ccd::FinalizeArgs f{L.split_rows, L.split_beg, L.partials, L.ptr, y, lambda, L.nsplit};
+        HIP_TRY(c, ccd::launch_finalize(f, c->stream));
+    }
+    return CCD_OK;
+}
+
+int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t nnz, const uint32_t* ptr,
+                  const uint32_t* idx, const float* val) {
+    L.rows = rows;
+    L.gdim = gdim;
+    L.nnz = nnz;
+    L.idx16 = gdim <= 65536;
+    HIP_TRY(c, dalloc(&L.ptr, rows + 1));
+    HIP_TRY(c, hipMemcpy(L.ptr, ptr, sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice));
+    if (L.idx16) {
+        std::vector<uint16_t> tmp(nnz);
+        for (uint64_t e = 0; e < nnz; ++e) tmp[e] = (uint16_t)idx[e];
+        uint16_t* d = nullptr;
+        HIP_TRY(c, dalloc(&d, nnz));
+        L.idx = d;
+        HIP_TRY(c, hipMemcpy(d, tmp.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
+    } else {
+        uint32_t* d = nullptr;
+        HIP_TRY(c, dalloc(&d, nnz));
+        L.idx = d;
+        HIP_TRY(c, hipMemcpy(d, idx, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, dalloc(&L.val, nnz));
+    HIP_TRY(c, dalloc(&L.val0, nnz));
+    HIP_TRY(c, hipMemcpy(L.val0, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(L.val, L.val0, sizeof(float) * nnz, hipMemcpyDeviceToDevice));
+
+    HostPlan P = make_plan(ptr, rows, c->plan);
+    if (P.split_beg.back() >= (1u << 24) || P.units.size() >= 0xffffffffull)
+        return set_err(c, CCD_ERR_INVALID, "work plan too large: raise ccd_plan_params.piece");
+    L.nunits = (uint32_t)P.units.size();
+    L.nsplit = (uint32_t)P.split_rows.size();
+    L.npieces = P.split_beg.back();
+    HIP_TRY(c, dalloc(&L.units, P.units.size()));
+    HIP_TRY(c, hipMemcpy(L.units, P.units.data(), sizeof(uint4) * P.units.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&L.rowlist, P.rowlist.size()));
+    if (!P.rowlist.empty())
+        HIP_TRY(c, hipMemcpy(L.rowlist, P.rowlist.data(), sizeof(uint32_t) * P.rowlist.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&L.split_rows, P.split_rows.size()));
+    if (!P.split_rows.empty())
+        HIP_TRY(c, hipMemcpy(L.split_rows, P.split_rows.data(), sizeof(uint32_t) * P.split_rows.size(),
+                             hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&L.split_beg, P.split_beg.size()));
+    HIP_TRY(c, hipMemcpy(L.split_beg, P.split_beg.data(), sizeof(uint32_t) * P.split_beg.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&L.partials, L.npieces));
+    return CCD_OK;
+}
+
+const char* check_compressed(int64_t rows, int64_t dim, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
+    if (ptr[0] != 0) return "pointer array must start at 0";
+    for (int64_t r = 0; r < rows; ++r)
+        if (ptr[r + 1] < ptr[r]) return "pointer array is not monotone";
+    if ((uint64_t)ptr[rows] != nnz) return "last pointer differs from nnz";
+    for (uint64_t e = 0; e < nnz; ++e)
+        if ((int64_t)idx[e] >= dim) return "index out of range";
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ccd_abi_version(void) { return CCD_ABI_VERSION; }
+
+int32_t ccd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void ccd_params_default(ccd_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->k = 10;
+    p->lambda = 0.1f;
+    p->maxiter = 5;
+    p->maxinneriter = 5;
+    p->variant = CCD_VARIANT_NATIVE;
+    p->precision = CCD_PREC_FP32;
+    p->order = CCD_ORDER_H_FIRST;
+    p->zero_first = 1;
+    p->rmse_every = 1;
+    p->profile = 0;
+    p->use_graph = 0;
+}
+
+int32_t ccd_create(int32_t device, ccd_ctx** out) {
+    if (!out) return CCD_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return CCD_ERR_HIP;
+    if (device < 0 || device >= count) return CCD_ERR_INVALID;
+    ccd_ctx* c = new (std::nothrow) ccd_ctx();
+    if (!c) return CCD_ERR_ALLOC;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CCD_ERR_HIP;
+    }
+    if (dalloc(&c->rmse_bsum, ccd::RMSE_BLOCKS) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return CCD_ERR_ALLOC;
+    }
+    *out = c;
+    return CCD_OK;
+}
+
+int32_t ccd_destroy(ccd_ctx* c) {
+    if (!c) return CCD_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_layout(c->csr);
+    free_layout(c->csc);
+    dfree(c->trow);
+    dfree(c->tcol);
+    dfree(c->tval);
+    dfree(c->W);
+    dfree(c->H);
+    dfree(c->u);
+    dfree(c->v);
+    dfree(c->rmse_bsum);
+    dfree(c->rmse_dev);
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CCD_OK;
+}
+
+const char* ccd_last_error(const ccd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t ccd_set_plan(ccd_ctx* c, const ccd_plan_params* pp) {
+    if (!c || !pp) return CCD_ERR_INVALID;
+    if (pp->elems_per_lane == 0 || pp->piece == 0 || pp->unit_elems == 0 || pp->split_threshold < pp->piece)
+        return set_err(c, CCD_ERR_INVALID, "plan: elems_per_lane, piece, unit_elems > 0 and split_threshold >= piece");
+    c->plan = *pp;
+    return CCD_OK;
+}
+
+int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_t* row_ptr, const uint32_t* col_idx,
+                   const float* csr_val, const uint32_t* col_ptr, const uint32_t* row_idx, const float* csc_val) {
+    if (!c) return CCD_ERR_INVALID;
+    if (m <= 0 || n <= 0 || m > 0x7fffffff || n > 0x7fffffff || nnz > 0xffffffffull)
+        return set_err(c, CCD_ERR_INVALID, "shape out of range (m, n < 2^31, nnz < 2^32)");
+    if (!row_ptr || !col_ptr || (nnz && (!col_idx || !csr_val || !row_idx || !csc_val)))
+        return set_err(c, CCD_ERR_INVALID, "null array");
+    if (const char* why = check_compressed(m, n, nnz, row_ptr, col_idx)) return set_err(c, CCD_ERR_INVALID, std::string("CSR: ") + why);
+    if (const char* why = check_compressed(n, m, nnz, col_ptr, row_idx)) return set_err(c, CCD_ERR_INVALID, std::string("CSC: ") + why);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_layout(c->csr);
+    free_layout(c->csc);
+    c->uploaded = false;
+    c->factors = false;
+    c->m = m;
+    c->n = n;
+    c->nnz = nnz;
+    int rc = upload_layout(c, c->csr, m, n, nnz, row_ptr, col_idx, csr_val);
+    if (rc) return rc;
+    rc = upload_layout(c, c->csc, n, m, nnz, col_ptr, row_idx, csc_val);
+    if (rc) return rc;
+    dfree(c->u);
+    dfree(c->v);
+    HIP_TRY(c, dalloc(&c->u, m));
+    HIP_TRY(c, dalloc(&c->v, n));
+    c->uploaded = true;
+    return CCD_OK;
+}
+
+int32_t ccd_upload_test(ccd_ctx* c, uint64_t nnz, const uint32_t* row, const uint32_t* col, const float* val) {
+    if (!c) return CCD_ERR_INVALID;
+    if (!c->uploaded) return set_err(c, CCD_ERR_STATE, "ccd_upload_test before ccd_upload");
+    if (nnz && (!row || !col || !val)) return set_err(c, CCD_ERR_INVALID, "null array");
+    for (uint64_t e = 0; e < nnz; ++e)
+        if ((int64_t)row[e] >= c->m || (int64_t)col[e] >= c->n) return set_err(c, CCD_ERR_INVALID, "test index out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+    dfree(c->trow);
+    dfree(c->tcol);
+    dfree(c->tval);
+    c->trow = c->tcol = nullptr;
+    c->tval = nullptr;
+    c->nnz_test = nnz;
+    HIP_TRY(c, dalloc(&c->trow, nnz));
+    HIP_TRY(c, dalloc(&c->tcol, nnz));
+    HIP_TRY(c, dalloc(&c->tval, nnz));
+    if (nnz) {
+        HIP_TRY(c, hipMemcpy(c->trow, row, 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->tcol, col, 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->tval, val, 4 * nnz, hipMemcpyHostToDevice));
+    }
+    return CCD_OK;
+}
+
+int32_t ccd_set_factors(ccd_ctx* c, uint32_t k, const float* W, const float* H) {
+    if (!c) return CCD_ERR_INVALID;
+    if (!c->uploaded) return set_err(c, CCD_ERR_STATE, "ccd_set_factors before ccd_upload");
+    if (k == 0 || !W || !H) return set_err(c, CCD_ERR_INVALID, "k must be > 0 and W, H non-null");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (k != c->k) {
+        dfree(c->W);
+        dfree(c->H);
+        c->W = c->H = nullptr;
+        HIP_TRY(c, dalloc(&c->W, (size_t)k * c->m));
+        HIP_TRY(c, dalloc(&c->H, (size_t)k * c->n));
+        c->k = k;
+    }
+    HIP_TRY(c, hipMemcpy(c->W, W, sizeof(float) * (size_t)k * c->m, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->H, H, sizeof(float) * (size_t)k * c->n, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->csr.val, c->csr.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, hipMemcpy(c->csc.val, c->csc.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
+    c->factors = true;
+    return CCD_OK;
+}
+
+int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
+    if (!c || !p) return CCD_ERR_INVALID;
+    if (!c->uploaded || !c->factors) return set_err(c, CCD_ERR_STATE, "ccd_run needs ccd_upload and ccd_set_factors");
+    if (p->k != c->k) return set_err(c, CCD_ERR_INVALID, "params.k differs from the k of ccd_set_factors");
+    if (p->maxiter < 0 || p->maxinneriter < 0 || !std::isfinite(p->lambda) || p->lambda < 0.0f)
+        return set_err(c, CCD_ERR_INVALID, "maxiter, maxinneriter >= 0 and lambda >= 0 finite");
+    if (p->variant != CCD_VARIANT_NATIVE && p->variant != CCD_VARIANT_BATCHING)
+        return set_err(c, CCD_ERR_UNSUPPORTED, "variant not available in this build");
+    if (p->precision != CCD_PREC_FP32) return set_err(c, CCD_ERR_UNSUPPORTED, "precision not available in this build");
+    if (p->order != CCD_ORDER_H_FIRST && p->order != CCD_ORDER_W_FIRST) return set_err(c, CCD_ERR_INVALID, "bad order");
+    HIP_TRY(c, hipSetDevice(c->device));
+
+    const bool hfirst = p->order == CCD_ORDER_H_FIRST;
+    DevLayout& F = hfirst ? c->csc : c->csr;
+    DevLayout& S = hfirst ? c->csr : c->csc;
+    float* XF = hfirst ? c->H : c->W;
+    float* XS = hfirst ? c->W : c->H;
+    float* vF = hfirst ? c->v : c->u;  // output of the first sweep
+    float* vS = hfirst ? c->u : c->v;
+    const int64_t dF = F.rows, dS = S.rows;
+    const uint32_t k = c->k;
+    const float lambda = p->lambda;
+    const int T = p->maxiter, Tin = p->maxinneriter;
+    const bool do_rmse = p->rmse_every && c->nnz_test > 0;
+
+    if (T > c->rmse_cap) {
+        dfree(c->rmse_dev);
+        c->rmse_dev = nullptr;
+        HIP_TRY(c, dalloc(&c->rmse_dev, T));
+        c->rmse_cap = T;
+    }
+    c->iter_rmse.assign(T, NAN);
+    c->prof = p->profile != 0;
+    c->recs.clear();
+    c->pool_used = 0;
+    for (int q = 0; q < CCD_K_COUNT; ++q) {
+        c->kind_ms[q] = 0.0;
+        c->kind_n[q] = 0;
+    }
+    c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, false, 1, 0, true);
+    c->kind_bytes[CCD_K_SWEEP_S] = layout_bytes(S, false, 1, 0, true);
+    c->kind_bytes[CCD_K_FUSED_F] = layout_bytes(F, true, 2, 2, true);
+    c->kind_bytes[CCD_K_FUSED_S] = layout_bytes(S, true, 3, 2, true);
+    c->kind_bytes[CCD_K_RESIDUAL] = 0.5 * (layout_bytes(F, true, 1, 1, false) + layout_bytes(S, true, 1, 1, false));
+    c->kind_bytes[CCD_K_FINALIZE] = 8.0 * (F.npieces + S.npieces) / 2.0;
+    c->kind_bytes[CCD_K_RMSE] = 12.0 * (double)c->nnz_test + 4.0 * (double)k * (double)(c->m + c->n);
+    c->kind_bytes[CCD_K_COPY] = 8.0 * (double)(c->m + c->n) / 2.0;
+
+    hipEvent_t loop_a = nullptr, loop_b = nullptr;
+    if (c->prof) {
+        loop_a = take_event(c);
+        loop_b = take_event(c);
+    }
+    if (p->zero_first) HIP_TRY(c, hipMemsetAsync(XF, 0, sizeof(float) * (size_t)k * dF, c->stream));
+    if (loop_a) HIP_TRY(c, hipEventRecord(loop_a, c->stream));
+
+    bool pending = false;  // rank `prev`'s final u v^T still to be subtracted
+    uint32_t prev = 0;
+    int rc;
+    size_t rec_mark = 0;
+    for (int oiter = 1; oiter <= T; ++oiter) {
+        for (uint32_t t = 0; t < k; ++t) {
+            const float* St = XS + (size_t)t * dS;
+            const float* Ft = XF + (size_t)t * dF;
+            const float* Sp = XS + (size_t)prev * dS;
+            const float* Fp = XF + (size_t)prev * dF;
+            const bool add = oiter > 1;
+            const bool fused = pending || add;
+            if (Tin >= 1) {
+                rc = do_pass(c, F, St, vF, Sp, Fp, St, Ft, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
+                if (rc) return rc;
+                rc = do_pass(c, S, vF, vS, Fp, Sp, Ft, St, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
+                if (rc) return rc;
+                for (int it = 2; it <= Tin; ++it) {
+                    rc = do_pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_F);
+                    if (rc) return rc;
+                    rc = do_pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_S);
+                    if (rc) return rc;
+                }
+                Timed tm(c, CCD_K_COPY);
+                HIP_TRY(c, hipMemcpyAsync(XS + (size_t)t * dS, vS, sizeof(float) * dS, hipMemcpyDeviceToDevice, c->stream));
+                HIP_TRY(c, hipMemcpyAsync(XF + (size_t)t * dF, vF, sizeof(float) * dF, hipMemcpyDeviceToDevice, c->stream));
+            } else if (fused) {
+                rc = do_pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, pending, add, false, lambda, CCD_K_RESIDUAL);
+                if (rc) return rc;
+                rc = do_pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, pending, add, false, lambda, CCD_K_RESIDUAL);
+                if (rc) return rc;
+            }
+            pending = true;
+            prev = t;
+        }
+        if (do_rmse) {
+            Timed tm(c, CCD_K_RMSE);
+            HIP_TRY(c, ccd::launch_rmse(c->trow, c->tcol, c->tval, c->nnz_test, c->W, c->H, c->m, c->n, k,
+                                        c->rmse_bsum, c->rmse_dev + (oiter - 1), c->stream));
+        }
+        if (cb) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            double rmse = NAN;
+            if (do_rmse) HIP_TRY(c, hipMemcpy(&rmse, c->rmse_dev + (oiter - 1), sizeof(double), hipMemcpyDeviceToHost));
+            double rank_t = 0.0, upd_t = 0.0, rmse_t = 0.0;
+            for (size_t q = rec_mark; q < c->recs.size(); ++q) {
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, c->recs[q].a, c->recs[q].b);
+                const int kd = c->recs[q].kind;
+                if (kd == CCD_K_RMSE) rmse_t += ms * 1e-3;
+                else if (kd == CCD_K_RESIDUAL || kd == CCD_K_COPY) upd_t += ms * 1e-3;
+                else rank_t += ms * 1e-3;
+            }
+            rec_mark = c->recs.size();
+            cb(user, oiter, rank_t, upd_t, rmse, rmse_t);
+        }
+    }
+    if (pending) {
+        const float* Sp = XS + (size_t)prev * dS;
+        const float* Fp = XF + (size_t)prev * dF;
+        rc = do_pass(c, F, nullptr, nullptr, Sp, Fp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
+        if (rc) return rc;
+        rc = do_pass(c, S, nullptr, nullptr, Fp, Sp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
+        if (rc) return rc;
+    }
+    if (loop_b) HIP_TRY(c, hipEventRecord(loop_b, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (do_rmse && T > 0) HIP_TRY(c, hipMemcpy(c->iter_rmse.data(), c->rmse_dev, sizeof(double) * T, hipMemcpyDeviceToHost));
+    c->loop_s = 0.0;
+    if (c->prof) {
+        for (const ProfRec& r : c->recs) {
+            float ms = 0.0f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, r.a, r.b));
+            c->kind_ms[r.kind] += ms;
+            c->kind_n[r.kind] += 1;
+        }
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, loop_a, loop_b));
+        c->loop_s = ms * 1e-3;
+    }
+    return CCD_OK;
+}
+
+int32_t ccd_get_factors(ccd_ctx* c, float* W, float* H) {
+    if (!c) return CCD_ERR_INVALID;
+    if (!c->factors) return set_err(c, CCD_ERR_STATE, "no factors");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (W) HIP_TRY(c, hipMemcpy(W, c->W, sizeof(float) * (size_t)c->k * c->m, hipMemcpyDeviceToHost));
+    if (H) HIP_TRY(c, hipMemcpy(H, c->H, sizeof(float) * (size_t)c->k * c->n, hipMemcpyDeviceToHost));
+    return CCD_OK;
+}
+
+int32_t ccd_get_residual(ccd_ctx* c, float* csr_val, float* csc_val) {
+    if (!c) return CCD_ERR_INVALID;
+    if (!c->uploaded) return set_err(c, CCD_ERR_STATE, "nothing uploaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (csr_val) HIP_TRY(c, hipMemcpy(csr_val, c->csr.val, 4 * c->nnz, hipMemcpyDeviceToHost));
+    if (csc_val) HIP_TRY(c, hipMemcpy(csc_val, c->csc.val, 4 * c->nnz, hipMemcpyDeviceToHost));
+    return CCD_OK;
+}
+
+int32_t ccd_rmse(ccd_ctx* c, double* out) {
+    if (!c || !out) return CCD_ERR_INVALID;
+    if (!c->factors) return set_err(c, CCD_ERR_STATE, "no factors");
+    if (c->nnz_test == 0) return set_err(c, CCD_ERR_STATE, "no test set");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->rmse_cap < 1) {
+        HIP_TRY(c, dalloc(&c->rmse_dev, 1));
+        c->rmse_cap = 1;
+    }
+    HIP_TRY(c, ccd::launch_rmse(c->trow, c->tcol, c->tval, c->nnz_test, c->W, c->H, c->m, c->n, c->k, c->rmse_bsum,
+                                c->rmse_dev, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->rmse_dev, sizeof(double), hipMemcpyDeviceToHost));
+    return CCD_OK;
+}
+
+int32_t ccd_iter_rmse(ccd_ctx* c, double* out, int32_t max_count) {
+    if (!c || !out) return CCD_ERR_INVALID;
+    const int32_t n = std::min<int32_t>(max_count, (int32_t)c->iter_rmse.size());
+    for (int32_t q = 0; q < n; ++q) out[q] = c->iter_rmse[q];
+    return n;
+}
+
+int32_t ccd_kernel_stats(ccd_ctx* c, int32_t kind, double* total_ms, uint64_t* launches, double* bytes_per_launch) {
+    if (!c || kind < 0 || kind >= CCD_K_COUNT) return CCD_ERR_INVALID;
+    if (total_ms) *total_ms = c->kind_ms[kind];
+    if (launches) *launches = c->kind_n[kind];
+    if (bytes_per_launch) *bytes_per_launch = c->kind_bytes[kind];
+    return CCD_OK;
+}
+
+int32_t ccd_loop_time(ccd_ctx* c, double* seconds) {
+    if (!c || !seconds) return CCD_ERR_INVALID;
+    *seconds = c->loop_s;
+    return CCD_OK;
+}
+
+int32_t ccd_copy_probe(ccd_ctx* c, uint64_t bytes, int32_t reps, double* ms_per_copy) {
+    if (!c || !ms_per_copy || bytes == 0 || bytes % 16 || reps <= 0) return CCD_ERR_INVALID;
+    HIP_TRY(c, hipSetDevice(c->device));
+    float4 *a = nullptr, *b = nullptr;
+    HIP_TRY(c, dalloc(&a, bytes / 16));
+    hipError_t e = dalloc(&b, bytes / 16);
+    if (e != hipSuccess) {
+        dfree(a);
+        return set_err(c, CCD_ERR_ALLOC, "copy probe allocation");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CCD_OK;
+    if (hipMemsetAsync(a, 0, bytes, c->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || ccd::launch_copy(a, b, bytes / 16, c->stream) != hipSuccess ||
+        hipEventRecord(e0, c->stream) != hipSuccess) {
+        rc = set_err(c, CCD_ERR_HIP, "copy probe launch");
+    } else {
+        for (int r = 0; r < reps && rc == CCD_OK; ++r)
+            if (ccd::launch_copy(a, b, bytes / 16, c->stream) != hipSuccess) rc = set_err(c, CCD_ERR_HIP, "copy probe launch");
+        float ms = 0.0f;
+        if (rc == CCD_OK && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = set_err(c, CCD_ERR_HIP, "copy probe timing");
+        *ms_per_copy = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(a);
+    dfree(b);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// Adapter: the reference's one-call solver signatures (src/cdmf.h:4-10) on
+// top of the C ABI of cdmf_hip.h.
+//
+// It uses only the public interface of SparseMatrix, TestData, MatData and
+// parameter, so it compiles against this project's host headers and equally
+// against the reference's own src/tools.h (then it links with the
+// reference's main.cpp and tools.cpp as a drop-in for cdmf_ocl_1.cpp,
+// cdmf_ocl_2.cpp and cdmf_csr5.cpp).
+//
+// Behaviour kept from the OpenCL drivers (src/cdmf_ocl_2.cpp:6-318):
+//   * H is zeroed before the solve and W is used as given;
+//   * one "[-INFO-] iteration num ..." line per outer iteration with the
+//     rank/update times and the FP64 test RMSE (cdmf_ocl_2.cpp:291-292);
+//     the CSR5 driver has no test set and prints it only with -q
+//     (cdmf_csr5.cpp:332-335);
+//   * "[INFO] OCL Training time: %lf s" (cdmf_ocl_2.cpp:297), the line the
+//     reference's scripts grep, and deltaT12 / deltaTAB for main's totals;
+//   * any device error prints a message and aborts, as CL_CHECK does
+//     (src/util.h:20-22).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "cdmf_hip.h"
+#include "tools.h"
+
+extern std::chrono::duration<double> deltaT12;
+extern std::chrono::duration<double> deltaTAB;
+
+#ifdef CDMF_HOST_PMF_H  // this project's parameter class carries the extension options
+#define CDMF_PARAM_ORDER(p) (p).order
+#define CDMF_PARAM_PRECISION(p) (p).precision
+#define CDMF_PARAM_GRAPH(p) (p).use_graph
+#define CDMF_PARAM_ZERO_FIRST(p) (p).zero_first
+#else
+#define CDMF_PARAM_ORDER(p) 0
+#define CDMF_PARAM_PRECISION(p) 0
+#define CDMF_PARAM_GRAPH(p) 0
+#define CDMF_PARAM_ZERO_FIRST(p) 1
+#endif
+
+namespace {
+
+void check(ccd_ctx* c, int rc, const char* what) {
+    if (rc != CCD_OK) {
+        fprintf(stderr, "Error \"%s\" (%d) in %s\n", c ? ccd_last_error(c) : "no context", rc, what);
+        abort();
+    }
+}
+
+struct IterPrint {
+    bool with_rmse;
+    bool verbose;
+    double rank_acc = 0.0, upd_acc = 0.0;
+};
+
+void on_iter(void* user, int32_t oiter, double rank_t, double upd_t, double rmse, double rmse_t) {
+    IterPrint* p = static_cast<IterPrint*>(user);
+    p->rank_acc += rank_t;
+    p->upd_acc += upd_t;
+    if (p->with_rmse) {
+        printf("[-INFO-] iteration num %d \trank_time %.4lf|%.4lf s \tupdate_time %.4lf|%.4lfs \tRMSE=%f time:%fs\n",
+               oiter, rank_t, p->rank_acc, upd_t, p->upd_acc, rmse, rmse_t);
+    } else if (p->verbose) {
+        printf("[VERBOSE] outter iteration num %d \trank_time %.4lf|%.4lf s \tupdate_time %.4lf|%.4lf s \n", oiter,
+               rank_t, p->rank_acc, upd_t, p->upd_acc);
+    }
+    fflush(stdout);
+}
+
+void flatten(const MatData& X, unsigned k, long n, std::vector<float>& out) {
+    out.resize((size_t)k * n);
+    for (unsigned t = 0; t < k; ++t)
+        for (long i = 0; i < n; ++i) out[(size_t)t * n + i] = (float)X[t][i];
+}
+
+void unflatten(const std::vector<float>& in, unsigned k, long n, MatData& X) {
+    for (unsigned t = 0; t < k; ++t)
+        for (long i = 0; i < n; ++i) X[t][i] = in[(size_t)t * n + i];
+}
+
+}  // namespace
+
+void cdmf_hip(SparseMatrix& R, MatData& W, MatData& H, TestData* T, parameter& param, int variant) {
+    auto tA = std::chrono::high_resolution_clock::now();
+    ccd_ctx* c = nullptr;
+    check(nullptr, ccd_create((int32_t)param.platform_id, &c), "ccd_create");
+    printf("[INFO] Connected!\n");
+    check(c, ccd_upload(c, R.rows, R.cols, R.nnz, R.get_csr_row_ptr(), R.get_csr_col_indx(), R.get_csr_val(),
+                        R.get_csc_col_ptr(), R.get_csc_row_indx(), R.get_csc_val()),
+          "ccd_upload");
+    if (T) check(c, ccd_upload_test(c, T->nnz, T->getTestRow(), T->getTestCol(), T->getTestVal()), "ccd_upload_test");
+    const unsigned k = param.k;
+    std::vector<float> Wf, Hf;
+    flatten(W, k, R.rows, Wf);
+    flatten(H, k, R.cols, Hf);
+    check(c, ccd_set_factors(c, k, Wf.data(), Hf.data()), "ccd_set_factors");
+    auto tB = std::chrono::high_resolution_clock::now();
+    deltaTAB = tB - tA;
+    std::cout << "[INFO] Initiating HIP Time: " << deltaTAB.count() << " s.\n";
+
+    ccd_params p;
+    ccd_params_default(&p);
+    p.k = k;
+    p.lambda = (float)param.lambda;
+    p.maxiter = param.maxiter;
+    p.maxinneriter = param.maxinneriter;
+    p.variant = variant;
+    p.precision = CDMF_PARAM_PRECISION(param);
+    p.order = CDMF_PARAM_ORDER(param);
+    p.zero_first = CDMF_PARAM_ZERO_FIRST(param);
+    p.use_graph = CDMF_PARAM_GRAPH(param);
+    p.rmse_every = T != nullptr;
+    p.profile = 1;
+    IterPrint pr{T != nullptr, param.verbose != 0};
+
+    std::cout << "------------------------------------------------------" << std::endl;
+    std::cout << "[INFO] Computing cdmf on the HIP device..." << std::endl;
+    auto t1 = std::chrono::high_resolution_clock::now();
+    check(c, ccd_run(c, &p, on_iter, &pr), "ccd_run");
+    auto t2 = std::chrono::high_resolution_clock::now();
+    deltaT12 = t2 - t1;
+    printf("[INFO] OCL Training time: %lf s\n", deltaT12.count());
+    if (variant == CCD_VARIANT_TILED) printf("[INFO] spmv time: %lf s | conversion time %lf s\n", pr.rank_acc, 0.0);
+
+    check(c, ccd_get_factors(c, Wf.data(), Hf.data()), "ccd_get_factors");
+    unflatten(Wf, k, R.rows, W);
+    unflatten(Hf, k, R.cols, H);
+    check(c, ccd_destroy(c), "ccd_destroy");
+}
+
+void cdmf_ocl_01(SparseMatrix& R, MatData& W_c, MatData& H_c, TestData& T, parameter& param, char filename[]) {
+    (void)filename;
+    cdmf_hip(R, W_c, H_c, &T, param, CCD_VARIANT_NATIVE);
+}
+
+void cdmf_ocl_02(SparseMatrix& R, MatData& W_c, MatData& H_c, TestData& T, parameter& param, char filename[]) {
+    (void)filename;
+    cdmf_hip(R, W_c, H_c, &T, param, CCD_VARIANT_BATCHING);
+}
+
+void cdmf_csr5(SparseMatrix& R, MatData& W, MatData& H, parameter& param, char filename[]) {
+    (void)filename;
+    cdmf_hip(R, W, H, nullptr, param, CCD_VARIANT_TILED);
+}

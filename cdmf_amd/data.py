@@ -1,0 +1,110 @@
+"""Synthetic data sets of the benchmark configurations and host-side init.
+
+Every generator is seeded; the same seed gives the same matrix on any
+machine and any thread count (xoshiro256** streams per row, see
+cdmf_amd/csrc/data/data.cpp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .datadir import Dataset
+
+
+@dataclass(frozen=True)
+class SolveConfig:
+    """Hyper-parameters of one CCD++ run (the CLI's -k -l -t -T)."""
+
+    k: int
+    lam: float
+    outer: int
+    inner: int
+
+
+# The small CPU-runnable case: 2,000 x 1,000, 50,000 distinct uniform
+# positions, integer ratings 1..5 uniform, 10 % held out; k=10, lambda=0.05,
+# 5 outer x 5 inner.  W = 0 and H ~ U[0, 1/sqrt(k)), so it runs W first.
+SMALL = dict(m=2000, n=1000, nnz_train=45000, nnz_test=5000, seed=0)
+SMALL_SOLVE = SolveConfig(k=10, lam=0.05, outer=5, inner=5)
+
+# The large case: 480,189 x 17,770 with 99,072,112 train + 1,408,395 test
+# ratings, power-law row degrees (min 1, max 17,000), column popularity
+# rank^-1.0, ratings 1..5 with P = [.05 .10 .29 .34 .22]; k=40, lambda=0.05,
+# 10 outer x 5 inner.
+NETFLIX = dict(m=480189, n=17770, nnz_train=99072112, nnz_test=1408395, max_row_degree=17000,
+               row_alpha=0.0, col_exponent=1.0, rating_kind=1, seed=1)
+NETFLIX_SOLVE = SolveConfig(k=40, lam=0.05, outer=10, inner=5)
+
+# The precision-study case: 138,493 x 26,744 with 20.0M ratings (10 % test),
+# power-law degrees with exponent 1.1 on the column side, half-star ratings.
+ML20M = dict(m=138493, n=26744, nnz_train=18000237, nnz_test=2000026, max_row_degree=9254,
+             row_alpha=0.0, col_exponent=1.1, rating_kind=2, seed=2)
+ML20M_SOLVE = SolveConfig(k=64, lam=0.05, outer=10, inner=5)
+
+
+def gen_uniform(m: int, n: int, nnz_train: int, nnz_test: int, seed: int = 0,
+                rating_kind: int = 0) -> Dataset:
+    cd = N.CDataset()
+    N.check_data(N.data_lib().cdmf_gen_uniform(m, n, nnz_train, nnz_test, rating_kind, seed, C.byref(cd)))
+    return N.dataset_from_c(cd)
+
+
+def gen_powerlaw(m: int, n: int, nnz_train: int, nnz_test: int, max_row_degree: int = 0,
+                 row_alpha: float = 0.0, col_exponent: float = 1.0, rating_kind: int = 1,
+                 seed: int = 0, threads: int = 0) -> Dataset:
+    spec = N.CGenSpec(m=m, n=n, nnz_train=nnz_train, nnz_test=nnz_test, max_row_degree=max_row_degree,
+                      row_alpha=row_alpha, col_exponent=col_exponent, rating_kind=rating_kind,
+                      threads=threads, seed=seed)
+    cd = N.CDataset()
+    N.check_data(N.data_lib().cdmf_gen_powerlaw(C.byref(spec), C.byref(cd)))
+    return N.dataset_from_c(cd)
+
+
+def small() -> Dataset:
+    return gen_uniform(**SMALL)
+
+
+def netflix(threads: int = 0) -> Dataset:
+    return gen_powerlaw(threads=threads, **NETFLIX)
+
+
+def ml20m(threads: int = 0) -> Dataset:
+    return gen_powerlaw(threads=threads, **ML20M)
+
+
+def load_dir(path: str) -> Dataset:
+    cd = N.CDataset()
+    N.check_data(N.data_lib().cdmf_load_dir(path.encode(), C.byref(cd)))
+    return N.dataset_from_c(cd)
+
+
+def write_dir(path: str, ds: Dataset) -> None:
+    import os
+    os.makedirs(path, exist_ok=True)
+    cd = N.dataset_to_c(ds)
+    N.check_data(N.data_lib().cdmf_write_dir(path.encode(), C.byref(cd)))
+
+
+def initial_col(k: int, n: int) -> np.ndarray:
+    """src/tools.cpp:349-357: rank-major [k][n] float32 init (srand(0), glibc rand)."""
+    X = np.empty((k, n), dtype=np.float32)
+    N.data_lib().cdmf_initial_col(N.ptr(X, C.c_float), k, n)
+    return X
+
+
+def init_uniform(shape, scale: float, seed: int) -> np.ndarray:
+    X = np.empty(shape, dtype=np.float32)
+    N.data_lib().cdmf_init_uniform(N.ptr(X, C.c_float), X.size, scale, seed)
+    return X
+
+
+def small_init(ds: Dataset, k: int = SMALL_SOLVE.k, seed: int = 0):
+    """W = 0 and H ~ U[0, 1/sqrt(k)) for the small case."""
+    W = np.zeros((k, ds.m), dtype=np.float32)
+    H = init_uniform((k, ds.n), 1.0 / math.sqrt(k), seed)
+    return W, H

@@ -1,0 +1,180 @@
+/*
+ * cdmf_hip.h -- C ABI of the MI355X (gfx950) CCD++ solver.
+ *
+ * Plain C: pointers, sizes and status codes only.  Host arrays passed in are
+ * borrowed for the duration of the call and copied; the context owns all
+ * device memory.  A context is bound to one HIP device and one stream and is
+ * not thread-safe: drive it from one host thread.
+ *
+ * Reference interfaces this boundary replaces (all C++ in the reference,
+ * src/cdmf.h:4-10, selected by -V in src/main.cpp:38-64):
+ *   void cdmf_ocl_01(SparseMatrix&, MatData& W, MatData& H, TestData&, parameter&, char kernel_file[])
+ *                                                     src/cdmf_ocl_1.cpp:6-423
+ *   void cdmf_ocl_02(SparseMatrix&, MatData& W, MatData& H, TestData&, parameter&, char kernel_file[])
+ *                                                     src/cdmf_ocl_2.cpp:6-318
+ *   void cdmf_csr5(SparseMatrix&, MatData& W, MatData& H, parameter&, char kernel_file[])
+ *                                                     src/cdmf_csr5.cpp:7-369
+ * Each of those is one call that uploads R, runs the outer/rank/inner loop
+ * and writes W, H back.  Here the same work is split into
+ *   ccd_create/ccd_destroy        OpenCL platform/context/queue set-up and release
+ *                                 (cdmf_ocl_2.cpp:11-21, 299-317)
+ *   ccd_upload                    the clCreateBuffer(COPY_HOST_PTR) uploads of the
+ *                                 CSR and CSC copies of R (cdmf_ocl_2.cpp:82-93)
+ *   ccd_upload_test               the test-set buffers of V1 (cdmf_ocl_1.cpp:121-140)
+ *   ccd_set_factors               the W/H uploads (cdmf_ocl_1.cpp:70-104)
+ *   ccd_run                       the oiter/t/inner loop with per-kernel timing
+ *                                 and the per-iteration RMSE line (cdmf_ocl_2.cpp:205-297)
+ *   ccd_get_factors               the final W/H read-back (cdmf_ocl_1.cpp:388-421)
+ *   ccd_get_residual              the residual the oracle leaves in R (cdmf_ref.cpp:120-121)
+ *   ccd_rmse                      calculate_rmse_directly on the device (tools.cpp:543-582,
+ *                                 GPU_rmse in kcode/ccd_1.cl:1-35)
+ * The C++ adapter cdmf_amd/csrc/host/cdmf_hip.cpp restores the one-call
+ * signatures above on top of these entry points.
+ */
+#ifndef CDMF_HIP_H
+#define CDMF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCD_ABI_VERSION 1
+
+/* Status codes: 0 = success, negative = error (message: ccd_last_error). */
+#define CCD_OK 0
+#define CCD_ERR_INVALID (-1)     /* bad argument or shape */
+#define CCD_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define CCD_ERR_ALLOC (-3)       /* device or host allocation failed */
+#define CCD_ERR_STATE (-4)       /* call out of order (e.g. run before upload) */
+#define CCD_ERR_UNSUPPORTED (-5) /* option not available in this build */
+
+typedef struct ccd_ctx ccd_ctx;
+
+/* Solver variants (-V).  1 and 2 select the degree-binned sweeps (the
+ * reference's "Native" and "Thread Batching" kernels have one replacement);
+ * 3 selects the tiled segmented-sum sweeps (the CSR5 "Load Balancing" path). */
+enum ccd_variant { CCD_VARIANT_NATIVE = 1, CCD_VARIANT_BATCHING = 2, CCD_VARIANT_TILED = 3 };
+
+/* Storage precision of W and H.  Residual values and all arithmetic stay FP32
+ * except CCD_PREC_FP64_SUMS, which accumulates each row's numerator and
+ * denominator in FP64 (the oracle-check mode). */
+enum ccd_precision { CCD_PREC_FP32 = 0, CCD_PREC_BF16 = 1, CCD_PREC_FP64_SUMS = 2 };
+
+/* Which factor is swept first inside a rank.  H_FIRST is the reference
+ * (v-sweep over the CSC copy, then the u-sweep, cdmf_ref.cpp:96-107).
+ * W_FIRST is the same algorithm on R^T with W and H exchanged. */
+enum ccd_order { CCD_ORDER_H_FIRST = 0, CCD_ORDER_W_FIRST = 1 };
+
+typedef struct ccd_params {
+    uint32_t k;           /* rank (-k); equals the k given to ccd_set_factors */
+    float lambda;         /* regularisation (-l) */
+    int32_t maxiter;      /* outer iterations (-t) */
+    int32_t maxinneriter; /* inner iterations (-T) */
+    int32_t variant;      /* enum ccd_variant */
+    int32_t precision;    /* enum ccd_precision */
+    int32_t order;        /* enum ccd_order */
+    int32_t zero_first;   /* 1: zero the first-swept factor at entry (the reference zeroes H,
+                             cdmf_ref.cpp:57-61); 0: keep it (the product W H^T must then be 0) */
+    int32_t rmse_every;   /* 1: test RMSE after every outer iteration (needs ccd_upload_test) */
+    int32_t profile;      /* 1: time every kernel with HIP events (ccd_kernel_stats) */
+    int32_t use_graph;    /* 1: replay each outer iteration from a captured HIP graph */
+    int32_t reserved[5];  /* zero */
+} ccd_params;
+
+/* Fills the reference defaults of src/pmf.h:4-26 (k=10, lambda=0.1, 5 outer,
+ * 5 inner, variant 1, FP32, H first, zero_first 1, RMSE every iteration). */
+void ccd_params_default(ccd_params* p);
+
+/* Called after each outer iteration when passed to ccd_run.  Times in
+ * seconds: rank_time = sweep kernels, update_time = residual-only kernels and
+ * copies (both from HIP events when profile = 1, else 0), rmse_time = the
+ * RMSE kernels; rmse is NaN when no test set is loaded.  Mirrors the line
+ * printed at cdmf_ref.cpp:135-136 / cdmf_ocl_2.cpp:291-292. */
+typedef void (*ccd_iter_cb)(void* user, int32_t oiter, double rank_time, double update_time, double rmse,
+                            double rmse_time);
+
+int32_t ccd_abi_version(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int32_t ccd_device_count(void);
+
+int32_t ccd_create(int32_t device, ccd_ctx** out);
+int32_t ccd_destroy(ccd_ctx* ctx);
+const char* ccd_last_error(const ccd_ctx* ctx);
+
+/* R as both compressed layouts: CSR (row_ptr[m+1], col_idx[nnz], csr_val[nnz])
+ * and CSC (col_ptr[n+1], row_idx[nnz], csc_val[nnz]) of the same matrix.
+ * Indices are validated (pointers monotone, ptr[last] == nnz, idx < dim). */
+int32_t ccd_upload(ccd_ctx* ctx, int64_t m, int64_t n, uint64_t nnz, const uint32_t* row_ptr,
+                   const uint32_t* col_idx, const float* csr_val, const uint32_t* col_ptr, const uint32_t* row_idx,
+                   const float* csc_val);
+
+/* COO test set (0-based).  nnz may be 0 (no RMSE). */
+int32_t ccd_upload_test(ccd_ctx* ctx, uint64_t nnz, const uint32_t* row, const uint32_t* col, const float* val);
+
+/* W rank-major [k][m], H rank-major [k][n] (the [t][i] indexing of MatData,
+ * src/pmf_util.h:7-8, flattened).  Resets the residual to R (the reference's
+ * start state, valid when W H^T = 0). */
+int32_t ccd_set_factors(ccd_ctx* ctx, uint32_t k, const float* W, const float* H);
+
+int32_t ccd_run(ccd_ctx* ctx, const ccd_params* params, ccd_iter_cb cb, void* user);
+
+int32_t ccd_get_factors(ccd_ctx* ctx, float* W, float* H);
+
+/* Residual values R - W H^T on the observed entries, in the caller's CSR and
+ * CSC orders (either pointer may be NULL). */
+int32_t ccd_get_residual(ccd_ctx* ctx, float* csr_val, float* csc_val);
+
+/* Test RMSE of the current factors, FP64 reduction on the device. */
+int32_t ccd_rmse(ccd_ctx* ctx, double* out);
+
+/* Per-outer-iteration test RMSEs of the last ccd_run (count = maxiter). */
+int32_t ccd_iter_rmse(ccd_ctx* ctx, double* out, int32_t max_count);
+
+/* Kernel classes timed when profile = 1. */
+enum ccd_kernel_kind {
+    CCD_K_SWEEP_F = 0,      /* plain sweep over the first-swept layout */
+    CCD_K_SWEEP_S = 1,      /* plain sweep over the second layout */
+    CCD_K_FUSED_F = 2,      /* residual -/+ fused with the first sweep, first layout */
+    CCD_K_FUSED_S = 3,      /* the same on the second layout */
+    CCD_K_RESIDUAL = 4,     /* residual-only update */
+    CCD_K_FINALIZE = 5,     /* split-row reductions */
+    CCD_K_RMSE = 6,         /* test RMSE */
+    CCD_K_COPY = 7,         /* factor copies */
+    CCD_K_COUNT = 8
+};
+
+/* Totals for one kernel class over the last ccd_run: summed duration (ms),
+ * launch count, and algorithmic HBM bytes per launch (the byte model of the
+ * class for the uploaded matrix: index, value and pointer streams plus one
+ * pass over each dense vector). */
+int32_t ccd_kernel_stats(ccd_ctx* ctx, int32_t kind, double* total_ms, uint64_t* launches,
+                         double* bytes_per_launch);
+
+/* Wall time of the last ccd_run's outer loop, seconds (between the first
+ * and last HIP event of the loop; 0 when profile = 0). */
+int32_t ccd_loop_time(ccd_ctx* ctx, double* seconds);
+
+/* Stream-copy calibration kernel: copies `bytes` (multiple of 16) between two
+ * device buffers `reps` times; returns the mean kernel time in ms.  Used to
+ * scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters. */
+int32_t ccd_copy_probe(ccd_ctx* ctx, uint64_t bytes, int32_t reps, double* ms_per_copy);
+
+/* Work-plan knobs of the binned sweeps; they only change speed, never results'
+ * tolerance class.  Applied at the next ccd_upload. */
+typedef struct ccd_plan_params {
+    uint32_t elems_per_lane;  /* target elements per lane when sizing sub-wave groups (default 4) */
+    uint32_t split_threshold; /* rows longer than this are split across waves (default 8192) */
+    uint32_t piece;           /* elements per piece of a split row (default 4096) */
+    uint32_t unit_elems;      /* target elements per wave work unit (default 4096) */
+} ccd_plan_params;
+
+int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDMF_HIP_H */

@@ -1,0 +1,243 @@
+"""Pin the CPU oracle before trusting it.
+
+* Bit-for-bit against fixtures written by the compiled reference
+  (tests/golden/make_golden.py runs the reference's cdmf_ref.cpp).
+* Functions with no reference counterpart (FP64 and BF16 twins, objective,
+  dual SpMV, FP64 sweep, FP64 residual) against mathematics: closed forms
+  recomputed independently in numpy/scipy, the transpose symmetry, a
+  monotone objective, bf16 representability.
+"""
+import hashlib
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from tests.helpers import golden_dataset, load_golden
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", ["simple", "tricky_simple"])
+def test_oracle_matches_reference_fixture(oracle_lib, name):
+    g = load_golden(name)
+    ds = golden_dataset(g)
+    k, lam, outer, inner = int(g["k"]), float(g["lam"]), int(g["outer"]), int(g["inner"])
+    for it in range(1, outer + 1):
+        W0 = oracle_lib.initial_col(k, ds.m)
+        H0 = oracle_lib.initial_col(k, ds.n)
+        W, H, rm, rcsr, rcsc = oracle_lib.ccd(ds, W0, H0, k, lam, it, inner)
+        assert np.array_equal(W, g["W"][it - 1]), "W differs at outer iteration %d" % it
+        assert np.array_equal(H, g["H"][it - 1]), "H differs at outer iteration %d" % it
+        assert rm[-1] == g["rmse"][it - 1]
+        assert np.array_equal(rm, g["rmse"][:it])
+    assert np.array_equal(rcsr, g["res_csr"])
+    assert np.array_equal(rcsc, g["res_csc"])
+
+
+def test_tricky_empty_column_is_zero(oracle_lib):
+    g = load_golden("tricky_simple")
+    ds = golden_dataset(g)
+    empty = np.flatnonzero(np.diff(ds.csc_ptr.astype(np.int64)) == 0)
+    assert empty.tolist() == [1]  # column 2 (1-based) has no training entry
+    assert np.all(g["H"][-1][:, empty] == 0.0)
+
+
+def _small():
+    from cdmf_amd import data as D
+    return D.small()
+
+
+def test_small_dataset_is_the_fixture_input():
+    g = load_golden("small_ref")
+    assert dataset_sha_local(_small()) == str(g["data_sha"])
+
+
+def dataset_sha_local(ds):
+    h = hashlib.sha256()
+    for a in (ds.csr_ptr, ds.csr_idx, ds.csr_val, ds.csc_ptr, ds.csc_idx, ds.csc_val, ds.test_row, ds.test_col,
+              ds.test_val):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def test_oracle_matches_reference_small_hfirst(oracle_lib):
+    g = load_golden("small_ref")
+    ds = _small()
+    k, lam, outer, inner = int(g["k"]), float(g["lam"]), int(g["outer"]), int(g["inner"])
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    for it in range(1, outer + 1):
+        W, H, rm, rcsr, rcsc = oracle_lib.ccd(ds, W0, H0, k, lam, it, inner, threads=4)
+        assert sha(W) == str(g["W_sha"][it - 1])
+        assert sha(H) == str(g["H_sha"][it - 1])
+    assert np.array_equal(rm, g["rmse"])
+    assert np.array_equal(W, g["W"]) and np.array_equal(H, g["H"])
+    assert sha(rcsr) == str(g["res_csr_sha"]) and sha(rcsc) == str(g["res_csc_sha"])
+
+
+def test_oracle_matches_reference_small_wfirst(oracle_lib):
+    from cdmf_amd import data as D
+    g = load_golden("small_wfirst")
+    ds = _small()
+    k, lam, outer, inner = int(g["k"]), float(g["lam"]), int(g["outer"]), int(g["inner"])
+    W0, H0 = D.small_init(ds, k)
+    assert sha(H0) == str(g["H0_sha"])
+    W, H, rm, rcsr, rcsc = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner, order=1)
+    assert np.array_equal(rm, g["rmse"])
+    assert np.array_equal(W, g["W"]) and np.array_equal(H, g["H"])
+    assert sha(rcsr) == str(g["res_csr_sha"]) and sha(rcsc) == str(g["res_csc_sha"])
+
+
+def test_product_initial_col_matches_oracle(oracle_lib):
+    from cdmf_amd import data as D
+    assert np.array_equal(D.initial_col(7, 33), oracle_lib.initial_col(7, 33))
+
+
+# ----------------------------------------------------------------- math pins
+
+def _dense(ds):
+    R = np.zeros((ds.m, ds.n))
+    rows = np.repeat(np.arange(ds.m), np.diff(ds.csr_ptr.astype(np.int64)))
+    R[rows, ds.csr_idx] = ds.csr_val
+    mask = np.zeros((ds.m, ds.n), dtype=bool)
+    mask[rows, ds.csr_idx] = True
+    return R, mask
+
+
+def _np_ccd_f64(R, mask, W, H, k, lam, outer, inner):
+    """Independent numpy restatement of CCD++ (dense, masked, FP64)."""
+    W, H = W.astype(np.float64).copy(), np.zeros_like(H, dtype=np.float64)
+    Rh = np.where(mask, R, 0.0)
+    nj, ni = mask.sum(0), mask.sum(1)
+    for oiter in range(1, outer + 1):
+        for t in range(k):
+            u, v = W[t].copy(), H[t].copy()
+            if oiter > 1:
+                Rh += np.where(mask, np.outer(W[t], H[t]), 0.0)
+            for _ in range(inner):
+                g = (u[:, None] * Rh).sum(0)
+                h = (mask * (u * u)[:, None]).sum(0)
+                v = np.where(nj > 0, g / np.maximum(lam * nj + h, 1e-300), 0.0)
+                g = (Rh * v[None, :]).sum(1)
+                h = (mask * (v * v)[None, :]).sum(1)
+                u = np.where(ni > 0, g / np.maximum(lam * ni + h, 1e-300), 0.0)
+            W[t], H[t] = u, v
+            Rh -= np.where(mask, np.outer(u, v), 0.0)
+    return W, H, Rh
+
+
+@pytest.mark.parametrize("name", ["simple", "tricky_simple"])
+def test_f64_twin_equals_numpy_ccd(oracle_lib, name):
+    g = load_golden(name)
+    ds = golden_dataset(g)
+    k, lam = 4, 0.05
+    W0 = oracle_lib.initial_col(k, ds.m)
+    H0 = oracle_lib.initial_col(k, ds.n)
+    W, H, rm, rcsr, _ = oracle_lib.ccd(ds, W0, H0, k, lam, 3, 2, dtype="f64")
+    R, mask = _dense(ds)
+    Wn, Hn, Rh = _np_ccd_f64(R, mask, W0, H0, k, lam, 3, 2)
+    np.testing.assert_allclose(W, Wn, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(H, Hn, rtol=1e-12, atol=1e-14)
+    rows = np.repeat(np.arange(ds.m), np.diff(ds.csr_ptr.astype(np.int64)))
+    np.testing.assert_allclose(rcsr, Rh[rows, ds.csr_idx], rtol=1e-12, atol=1e-13)
+
+
+def test_f32_close_to_f64_on_small(oracle_lib):
+    ds = _small()
+    k = 10
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    _, _, r32, _, _ = oracle_lib.ccd(ds, W0, H0, k, 0.05, 3, 3)
+    _, _, r64, _, _ = oracle_lib.ccd(ds, W0, H0, k, 0.05, 3, 3, dtype="f64")
+    assert np.max(np.abs(r32 - r64)) < 1e-4
+
+
+def test_wfirst_is_hfirst_on_transpose(oracle_lib):
+    """order=1 on R equals order=0 on R^T with W and H exchanged, bit for bit."""
+    ds = _small()
+    k = 5
+    W0 = np.zeros((k, ds.m), np.float32)
+    H0 = oracle_lib.initial_col(k, ds.n)
+    W1, H1, r1, a1, b1 = oracle_lib.ccd(ds, W0, H0, k, 0.05, 2, 3, order=1)
+    H2, W2, r2, b2, a2 = oracle_lib.ccd(ds.transpose(), H0, W0, k, 0.05, 2, 3, order=0)
+    assert np.array_equal(W1, W2) and np.array_equal(H1, H2) and np.array_equal(r1, r2)
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+
+
+def test_objective_non_increasing(oracle_lib):
+    ds = _small()
+    k, lam = 6, 0.05
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    objs = []
+    for it in range(1, 5):
+        W, H, _, rcsr, _ = oracle_lib.ccd(ds, W0, H0, k, lam, it, 2)
+        objs.append(oracle_lib.objective(ds, rcsr, W, H, lam))
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(objs, objs[1:])), objs
+    # objective recomputed independently
+    R, mask = _dense(ds)
+    Rh = np.where(mask, R - W.astype(np.float64).T @ H.astype(np.float64), 0.0)
+    ref = (Rh ** 2).sum() + lam * ((mask.sum(1) * (W.astype(np.float64) ** 2).sum(0)).sum()
+                                   + (mask.sum(0) * (H.astype(np.float64) ** 2).sum(0)).sum())
+    assert math.isclose(objs[-1], ref, rel_tol=1e-6)
+
+
+def test_bf16_twin(oracle_lib):
+    ds = _small()
+    k = 4
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    W, H, rb, _, _ = oracle_lib.ccd(ds, W0, H0, k, 0.05, 2, 2, dtype="bf16")
+    for X in (W, H):
+        assert np.all((X.view(np.uint32) & 0xFFFF) == 0)  # every stored value is a bf16
+    _, _, rf, _, _ = oracle_lib.ccd(ds, W0, H0, k, 0.05, 2, 2)
+    assert np.max(np.abs(rb - rf)) < 2e-2
+    # rounding rule: nearest, ties to even
+    assert oracle_lib.bf16(1.0 + 2 ** -8) == 1.0
+    assert oracle_lib.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
+    assert oracle_lib.bf16(1.0 + 2 ** -8 + 2 ** -20) == 1.0 + 2 ** -7
+
+
+def test_dual_spmv_and_sweep_against_scipy(oracle_lib):
+    ds = _small()
+    A = sp.csr_matrix((ds.csr_val.astype(np.float64), ds.csr_idx, ds.csr_ptr.astype(np.int64)), shape=(ds.m, ds.n))
+    P = A.copy()
+    P.data[:] = 1.0
+    x = np.random.default_rng(0).random(ds.n).astype(np.float32)
+    yt, yb = oracle_lib.dual_spmv(ds.m, ds.csr_ptr, ds.csr_idx, ds.csr_val, x)
+    np.testing.assert_allclose(yt, A @ x.astype(np.float64), rtol=1e-12)
+    np.testing.assert_allclose(yb, P @ (x.astype(np.float64) ** 2), rtol=1e-12)
+    y = oracle_lib.sweep_f64(ds.m, ds.csr_ptr, ds.csr_idx, ds.csr_val, x, 0.05)
+    d = np.diff(ds.csr_ptr.astype(np.int64))
+    lam = float(np.float32(0.05))  # the oracle takes lambda as a float
+    ref = np.where(d > 0, yt / np.maximum(lam * d + yb, 1e-300), 0.0)
+    np.testing.assert_allclose(y, ref, rtol=1e-12)
+
+
+def test_residual_f64_against_dense(oracle_lib):
+    ds = _small()
+    k = 3
+    rng = np.random.default_rng(1)
+    W = rng.random((k, ds.m)).astype(np.float32)
+    H = rng.random((k, ds.n)).astype(np.float32)
+    r = oracle_lib.residual_f64(ds, W, H)
+    rows = np.repeat(np.arange(ds.m), np.diff(ds.csr_ptr.astype(np.int64)))
+    ref = ds.csr_val - (W[:, rows].astype(np.float64) * H[:, ds.csr_idx].astype(np.float64)).sum(0)
+    np.testing.assert_allclose(r, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_rmse_counts_nan_terms(oracle_lib):
+    ds = _small()
+    k = 2
+    W = np.ones((k, ds.m), np.float32)
+    H = np.ones((k, ds.n), np.float32)
+    base = oracle_lib.rmse(W, H, ds)
+    ref = math.sqrt(np.mean((2.0 - ds.test_val.astype(np.float64)) ** 2))
+    assert math.isclose(base, ref, rel_tol=1e-14)
+    H2 = H.copy()
+    H2[0, ds.test_col[0]] = np.nan
+    hit = np.count_nonzero(ds.test_col == ds.test_col[0])
+    d = (2.0 - ds.test_val.astype(np.float64)) ** 2
+    d[ds.test_col == ds.test_col[0]] = 0.0
+    assert math.isclose(oracle_lib.rmse(W, H2, ds), math.sqrt(d.sum() / ds.nnz_test), rel_tol=1e-14)
+    assert hit >= 1
