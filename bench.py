@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
+DEFAULT_VARIANT = 1
 
 
 def log(msg):
@@ -87,6 +88,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="the 2,000 x 1,000 case instead (quick check)")
+    ap.add_argument("--variant", type=int, default=DEFAULT_VARIANT, help="1/2 = degree-binned, 3 = tiled sweeps")
+    ap.add_argument("--sigma", type=int, default=16, help="elements per lane of a tile (tiled sweeps)")
+    ap.add_argument("--probe", action="store_true",
+                    help="run a 256 MiB stream-copy kernel first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
@@ -115,6 +120,10 @@ def main():
     H0 = np.zeros((k, ds.n), np.float32)
 
     s = S.Solver(0)
+    s.set_plan(sigma=args.sigma)
+    if args.probe:
+        ms = s.copy_probe(256 << 20, reps=5)
+        log("copy probe: 256 MiB in %.1f us (%.0f GB/s read+write)" % (ms * 1e3, 2 * (256 << 20) / (ms * 1e-3) / 1e9))
     t = time.perf_counter()
     s.upload(ds)
     log("uploaded in %.1fs" % (time.perf_counter() - t))
@@ -122,12 +131,12 @@ def main():
     if args.warmup > 0:
         s.set_factors(W0, H0)
         t = time.perf_counter()
-        s.run(k, lam, args.warmup, inner, profile=False)
+        s.run(k, lam, args.warmup, inner, variant=args.variant, profile=False)
         log("warmup %d outer iteration(s) in %.2fs" % (args.warmup, time.perf_counter() - t))
 
     s.set_factors(W0, H0)
     t0 = time.perf_counter()
-    res = s.run(k, lam, args.steps, inner, profile=True)  # returns after the stream is drained
+    res = s.run(k, lam, args.steps, inner, variant=args.variant, profile=True)  # returns after the stream drains
     t1 = time.perf_counter()
     wall = t1 - t0
     log("timed %d outer iterations in %.3fs; test RMSE %s" % (args.steps, wall, ["%.6f" % r for r in res.rmse]))
@@ -145,13 +154,14 @@ def main():
     traffic = load_traffic()
     roofline = {
         "bound": "hbm",
-        "kernel": {"sweep_f": "sweep (CSC, v update)", "sweep_s": "sweep (CSR, u update)",
-                   "fused_f": "fused residual+sweep (CSC)", "fused_s": "fused residual+sweep (CSR)"}[dom],
+        "kernel": ("tiled " if args.variant == 3 else "binned ") + {
+            "sweep_f": "sweep (CSC, v update)", "sweep_s": "sweep (CSR, u update)",
+            "fused_f": "fused residual+sweep (CSC)", "fused_s": "fused residual+sweep (CSR)"}[dom],
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get(dom, {}).get("hbm_bytes_per_launch") if traffic else None,
+        "traffic": traffic.get("v%d" % args.variant, {}).get(dom, {}).get("hbm_bytes_per_launch") if traffic else None,
         "algorithmic_bytes_per_launch": d["bytes_per_launch"],
         "avg_launch_us": round(avg_ms * 1e3, 2),
     }
@@ -172,7 +182,8 @@ def main():
         "config": {"workload": "CCD++ rank-one loop, %d x %d, nnz=%d, k=%d, lambda=%g, %d outer x %d inner"
                                % (ds.m, ds.n, ds.nnz, k, lam, args.steps, inner),
                    "m": ds.m, "n": ds.n, "nnz": ds.nnz, "nnz_test": ds.nnz_test, "k": k, "lambda": lam,
-                   "outer": args.steps, "inner": inner, "seed": spec["seed"]},
+                   "outer": args.steps, "inner": inner, "seed": spec["seed"], "variant": args.variant,
+                   "sigma": args.sigma if args.variant == 3 else None},
         "roofline": roofline,
         "test_rmse": res.rmse,
         "loop_time_s": s.loop_time(),

@@ -32,6 +32,7 @@ HIP_FLAGS = [
 
 HIP_SOURCES = [
     "kernels/ccd_kernels.hip",
+    "kernels/tiled_kernels.hip",
     "ccd_abi.hip",
 ]
 HOST_SOURCES = [
@@ -78,7 +79,9 @@ def build_cli() -> str:
 def build_oracle() -> None:
     if shutil.which("make") is None:
         raise RuntimeError("make not found")
-    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    # the reference-side targets are skipped by the Makefile where the
+    # reference checkout is absent
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all", "refmain"])
 
 
 def build_all() -> None:

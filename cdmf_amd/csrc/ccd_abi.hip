@@ -26,6 +26,26 @@
 
 namespace {
 
+// Tiled copy of one layout for the tiled sweeps (see ccd_kernels.h).  The
+// index array is tiled once; residual values are tiled at the start of each
+// tiled run and written back to the layout's canonical order at its end.
+struct TiledLayout {
+    int sigma = 0;
+    uint32_t ntiles = 0;
+    void* tidx = nullptr;
+    float* tval = nullptr;
+    uint32_t* flags = nullptr;
+    uint16_t* seg0 = nullptr;
+    uint32_t* row0 = nullptr;
+    uint32_t* segoff = nullptr;
+    uint32_t* segrows = nullptr;
+    uint8_t* open = nullptr;
+    float2* head = nullptr;
+    float2* tail = nullptr;
+    uint32_t* span_rows = nullptr;
+    uint32_t nspan = 0;
+};
+
 struct DevLayout {
     int64_t rows = 0;   // rows of this layout (n for CSC, m for CSR)
     int64_t gdim = 0;   // length of the gathered vector (m for CSC, n for CSR)
@@ -43,6 +63,8 @@ struct DevLayout {
     uint32_t nsplit = 0;
     float2* partials = nullptr;
     uint32_t npieces = 0;
+    std::vector<uint32_t> hptr;  // host copy of ptr (tile planning)
+    TiledLayout tl;
 };
 
 struct ProfRec {
@@ -57,6 +79,9 @@ struct ccd_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool uploaded = false, factors = false;
+    // true once a run has left the residual at R - W H^T: the next run then
+    // adds each rank back from its first outer iteration on
+    bool continued = false;
     int64_t m = 0, n = 0;
     uint64_t nnz = 0;
     DevLayout csr, csc;
@@ -73,7 +98,7 @@ struct ccd_ctx {
     double* rmse_dev = nullptr;  // one slot per outer iteration
     int32_t rmse_cap = 0;
     std::vector<double> iter_rmse;
-    ccd_plan_params plan{4, 8192, 4096, 4096};
+    ccd_plan_params plan{4, 8192, 4096, 4096, 16};
     // profiling
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -109,7 +134,23 @@ void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
+void free_tiled(TiledLayout& T) {
+    dfree(T.tidx);
+    dfree(T.tval);
+    dfree(T.flags);
+    dfree(T.seg0);
+    dfree(T.row0);
+    dfree(T.segoff);
+    dfree(T.segrows);
+    dfree(T.open);
+    dfree(T.head);
+    dfree(T.tail);
+    dfree(T.span_rows);
+    T = TiledLayout{};
+}
+
 void free_layout(DevLayout& L) {
+    free_tiled(L.tl);
     dfree(L.ptr);
     dfree(L.idx);
     dfree(L.val);
@@ -243,12 +284,16 @@ int do_pass(ccd_ctx* c, DevLayout& L, const float* x, float* y, const float* xs,
     return CCD_OK;
 }
 
+using PassFn = int (*)(ccd_ctx*, DevLayout&, const float*, float*, const float*, const float*, const float*,
+                      const float*, bool, bool, bool, float, int);
+
 int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t nnz, const uint32_t* ptr,
                   const uint32_t* idx, const float* val) {
     L.rows = rows;
     L.gdim = gdim;
     L.nnz = nnz;
     L.idx16 = gdim <= 65536;
+    L.hptr.assign(ptr, ptr + rows + 1);
     HIP_TRY(c, dalloc(&L.ptr, rows + 1));
     HIP_TRY(c, hipMemcpy(L.ptr, ptr, sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice));
     if (L.idx16) {
@@ -287,6 +332,106 @@ int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t
     HIP_TRY(c, dalloc(&L.split_beg, P.split_beg.size()));
     HIP_TRY(c, hipMemcpy(L.split_beg, P.split_beg.data(), sizeof(uint32_t) * P.split_beg.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, dalloc(&L.partials, L.npieces));
+    return CCD_OK;
+}
+
+template <class T>
+int upload_vec(ccd_ctx* c, T** dst, const std::vector<T>& v) {
+    HIP_TRY(c, dalloc(dst, v.size()));
+    if (!v.empty()) HIP_TRY(c, hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return CCD_OK;
+}
+
+// Tile descriptors from the row pointer (host, O(rows + tiles)), index array
+// tiled on the device.
+int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
+    if (L.tl.sigma == sigma) return CCD_OK;
+    free_tiled(L.tl);
+    TiledLayout& T = L.tl;
+    const uint64_t TS = 64ull * (uint64_t)sigma, nnz = L.nnz;
+    const uint64_t ntiles = (nnz + TS - 1) / TS;
+    if (ntiles >= 0xffffffffull / 64) return set_err(c, CCD_ERR_INVALID, "too many tiles");
+    const uint32_t* ptr = L.hptr.data();
+    const int64_t rows = L.rows;
+    std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows, span;
+    std::vector<uint16_t> seg0(ntiles * 64);
+    std::vector<uint8_t> open(ntiles);
+    std::vector<uint32_t> emp(rows + 1, 0u);
+    for (int64_t r = 0; r < rows; ++r) {
+        const bool empty = ptr[r + 1] == ptr[r];
+        emp[r + 1] = emp[r] + (empty ? 1u : 0u);
+        if (empty) continue;
+        const uint64_t e = ptr[r], rem = e % TS;
+        flags[(e / TS) * 64 + rem / sigma] |= 1u << (rem % sigma);
+        if (e / TS != (ptr[r + 1] - 1ull) / TS) span.push_back((uint32_t)r);
+    }
+    int64_t r = 0;
+    for (uint64_t t = 0; t < ntiles; ++t) {
+        const uint64_t tbase = t * TS;
+        while ((uint64_t)ptr[r + 1] <= tbase) ++r;
+        row0[t] = (uint32_t)r;
+        uint8_t o = (uint64_t)ptr[r] < tbase ? 1 : 0;
+        const uint64_t tend = std::min<uint64_t>(tbase + TS, nnz) - 1;
+        int64_t r2 = r;
+        while ((uint64_t)ptr[r2 + 1] <= tend) ++r2;
+        if ((uint64_t)ptr[r2 + 1] > tbase + TS) o |= 2;
+        open[t] = o;
+        if (r2 > r + 1 && emp[r2] - emp[r + 1] > 0) {
+            segoff[t] = (uint32_t)segrows.size();
+            for (int64_t q = r; q <= r2; ++q)
+                if (ptr[q + 1] > ptr[q]) segrows.push_back((uint32_t)q);
+        } else {
+            segoff[t] = 0xffffffffu;
+        }
+        const uint32_t start0 = flags[t * 64] & 1u;
+        uint32_t cum = 0;
+        for (int l = 0; l < 64; ++l) {
+            const uint32_t f = flags[t * 64 + l];
+            seg0[t * 64 + l] = (uint16_t)(l == 0 ? 0u : cum - start0 + (f & 1u));
+            cum += (uint32_t)__builtin_popcount(f);
+        }
+    }
+    T.sigma = sigma;
+    T.ntiles = (uint32_t)ntiles;
+    T.nspan = (uint32_t)span.size();
+    int rc;
+    if ((rc = upload_vec(c, &T.flags, flags)) || (rc = upload_vec(c, &T.seg0, seg0)) ||
+        (rc = upload_vec(c, &T.row0, row0)) || (rc = upload_vec(c, &T.segoff, segoff)) ||
+        (rc = upload_vec(c, &T.segrows, segrows)) || (rc = upload_vec(c, &T.open, open)) ||
+        (rc = upload_vec(c, &T.span_rows, span)))
+        return rc;
+    const uint64_t padded = ntiles * TS;
+    HIP_TRY(c, dalloc(&T.tval, padded));
+    HIP_TRY(c, dalloc(&T.head, ntiles));
+    HIP_TRY(c, dalloc(&T.tail, ntiles));
+    if (L.idx16) {
+        uint16_t* d = nullptr;
+        HIP_TRY(c, dalloc(&d, padded));
+        T.tidx = d;
+    } else {
+        uint32_t* d = nullptr;
+        HIP_TRY(c, dalloc(&d, padded));
+        T.tidx = d;
+    }
+    if (ntiles) HIP_TRY(c, ccd::launch_retile_idx(L.idx, T.tidx, L.idx16, nnz, T.ntiles, sigma, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return CCD_OK;
+}
+
+int do_tiled_pass(ccd_ctx* c, DevLayout& L, const float* x, float* y, const float* xs, const float* ys,
+                  const float* xa, const float* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
+    TiledLayout& T = L.tl;
+    ccd::TiledArgs a{L.ptr, T.tidx, T.tval, x, y, xs, ys, xa, ya, T.flags, T.seg0, T.row0, T.segoff, T.segrows,
+                     T.open, T.head, T.tail, L.nnz, T.ntiles, lambda};
+    {
+        Timed tm(c, kind);
+        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, L.idx16, sub, add, sweep, c->stream));
+    }
+    if (sweep && T.nspan) {
+        Timed tm(c, CCD_K_FINALIZE);
+        ccd::SpanArgs sa{T.span_rows, L.ptr, T.head, T.tail, y, T.nspan, 64u * (uint32_t)T.sigma, lambda};
+        HIP_TRY(c, ccd::launch_span(sa, c->stream));
+    }
     return CCD_OK;
 }
 
@@ -377,6 +522,8 @@ int32_t ccd_set_plan(ccd_ctx* c, const ccd_plan_params* pp) {
     if (!c || !pp) return CCD_ERR_INVALID;
     if (pp->elems_per_lane == 0 || pp->piece == 0 || pp->unit_elems == 0 || pp->split_threshold < pp->piece)
         return set_err(c, CCD_ERR_INVALID, "plan: elems_per_lane, piece, unit_elems > 0 and split_threshold >= piece");
+    if (pp->sigma != 8 && pp->sigma != 16 && pp->sigma != 32)
+        return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8, 16 or 32");
     c->plan = *pp;
     return CCD_OK;
 }
@@ -454,6 +601,7 @@ int32_t ccd_set_factors(ccd_ctx* c, uint32_t k, const float* W, const float* H) 
     HIP_TRY(c, hipMemcpy(c->csr.val, c->csr.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
     HIP_TRY(c, hipMemcpy(c->csc.val, c->csc.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
     c->factors = true;
+    c->continued = false;
     return CCD_OK;
 }
 
@@ -463,10 +611,12 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     if (p->k != c->k) return set_err(c, CCD_ERR_INVALID, "params.k differs from the k of ccd_set_factors");
     if (p->maxiter < 0 || p->maxinneriter < 0 || !std::isfinite(p->lambda) || p->lambda < 0.0f)
         return set_err(c, CCD_ERR_INVALID, "maxiter, maxinneriter >= 0 and lambda >= 0 finite");
-    if (p->variant != CCD_VARIANT_NATIVE && p->variant != CCD_VARIANT_BATCHING)
-        return set_err(c, CCD_ERR_UNSUPPORTED, "variant not available in this build");
+    if (p->variant != CCD_VARIANT_NATIVE && p->variant != CCD_VARIANT_BATCHING && p->variant != CCD_VARIANT_TILED)
+        return set_err(c, CCD_ERR_INVALID, "variant must be 1, 2 or 3");
     if (p->precision != CCD_PREC_FP32) return set_err(c, CCD_ERR_UNSUPPORTED, "precision not available in this build");
     if (p->order != CCD_ORDER_H_FIRST && p->order != CCD_ORDER_W_FIRST) return set_err(c, CCD_ERR_INVALID, "bad order");
+    if (p->zero_first && c->continued)
+        return set_err(c, CCD_ERR_INVALID, "zero_first on a run that continues a previous one (call ccd_set_factors first)");
     HIP_TRY(c, hipSetDevice(c->device));
 
     const bool hfirst = p->order == CCD_ORDER_H_FIRST;
@@ -505,6 +655,18 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     c->kind_bytes[CCD_K_RMSE] = 12.0 * (double)c->nnz_test + 4.0 * (double)k * (double)(c->m + c->n);
     c->kind_bytes[CCD_K_COPY] = 8.0 * (double)(c->m + c->n) / 2.0;
 
+    const bool tiled = p->variant == CCD_VARIANT_TILED;
+    const PassFn pass = tiled ? do_tiled_pass : do_pass;
+    if (tiled) {
+        int rc0 = build_tiled(c, F, (int)c->plan.sigma);
+        if (!rc0) rc0 = build_tiled(c, S, (int)c->plan.sigma);
+        if (rc0) return rc0;
+        // residual into tile order; rows without entries keep y = 0
+        if (F.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(F.val, F.tl.tval, F.nnz, F.tl.ntiles, F.tl.sigma, true, c->stream));
+        if (S.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(S.val, S.tl.tval, S.nnz, S.tl.ntiles, S.tl.sigma, true, c->stream));
+        HIP_TRY(c, ccd::launch_fill(vF, 0.0f, (uint64_t)dF, c->stream));
+        HIP_TRY(c, ccd::launch_fill(vS, 0.0f, (uint64_t)dS, c->stream));
+    }
     hipEvent_t loop_a = nullptr, loop_b = nullptr;
     if (c->prof) {
         loop_a = take_event(c);
@@ -523,26 +685,26 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
             const float* Ft = XF + (size_t)t * dF;
             const float* Sp = XS + (size_t)prev * dS;
             const float* Fp = XF + (size_t)prev * dF;
-            const bool add = oiter > 1;
+            const bool add = oiter > 1 || c->continued;
             const bool fused = pending || add;
             if (Tin >= 1) {
-                rc = do_pass(c, F, St, vF, Sp, Fp, St, Ft, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
+                rc = pass(c, F, St, vF, Sp, Fp, St, Ft, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
                 if (rc) return rc;
-                rc = do_pass(c, S, vF, vS, Fp, Sp, Ft, St, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
+                rc = pass(c, S, vF, vS, Fp, Sp, Ft, St, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
                 if (rc) return rc;
                 for (int it = 2; it <= Tin; ++it) {
-                    rc = do_pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_F);
+                    rc = pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_F);
                     if (rc) return rc;
-                    rc = do_pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_S);
+                    rc = pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_S);
                     if (rc) return rc;
                 }
                 Timed tm(c, CCD_K_COPY);
                 HIP_TRY(c, hipMemcpyAsync(XS + (size_t)t * dS, vS, sizeof(float) * dS, hipMemcpyDeviceToDevice, c->stream));
                 HIP_TRY(c, hipMemcpyAsync(XF + (size_t)t * dF, vF, sizeof(float) * dF, hipMemcpyDeviceToDevice, c->stream));
             } else if (fused) {
-                rc = do_pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, pending, add, false, lambda, CCD_K_RESIDUAL);
+                rc = pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, pending, add, false, lambda, CCD_K_RESIDUAL);
                 if (rc) return rc;
-                rc = do_pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, pending, add, false, lambda, CCD_K_RESIDUAL);
+                rc = pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, pending, add, false, lambda, CCD_K_RESIDUAL);
                 if (rc) return rc;
             }
             pending = true;
@@ -573,13 +735,18 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     if (pending) {
         const float* Sp = XS + (size_t)prev * dS;
         const float* Fp = XF + (size_t)prev * dF;
-        rc = do_pass(c, F, nullptr, nullptr, Sp, Fp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
+        rc = pass(c, F, nullptr, nullptr, Sp, Fp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
         if (rc) return rc;
-        rc = do_pass(c, S, nullptr, nullptr, Fp, Sp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
+        rc = pass(c, S, nullptr, nullptr, Fp, Sp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
         if (rc) return rc;
     }
     if (loop_b) HIP_TRY(c, hipEventRecord(loop_b, c->stream));
+    if (tiled) {
+        if (F.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(F.tl.tval, F.val, F.nnz, F.tl.ntiles, F.tl.sigma, false, c->stream));
+        if (S.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(S.tl.tval, S.val, S.nnz, S.tl.ntiles, S.tl.sigma, false, c->stream));
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (T > 0) c->continued = true;
     if (do_rmse && T > 0) HIP_TRY(c, hipMemcpy(c->iter_rmse.data(), c->rmse_dev, sizeof(double) * T, hipMemcpyDeviceToHost));
     c->loop_s = 0.0;
     if (c->prof) {

@@ -23,7 +23,10 @@
 #include <vector>
 
 #include "cdmf_hip.h"
-#include "tools.h"
+// Angle brackets: the -I path picks this project's host headers or the
+// reference's src/tools.h; a quoted include would always find the copy
+// next to this file.
+#include <tools.h>
 
 extern std::chrono::duration<double> deltaT12;
 extern std::chrono::duration<double> deltaTAB;

@@ -43,9 +43,9 @@ __device__ __forceinline__ void strided_range(const SweepArgs& a, const IdxT* __
         uint32_t c[4];
         float r[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) c[q] = idx[p + q * step];
+        for (int q = 0; q < 4; ++q) c[q] = __builtin_nontemporal_load(idx + p + q * step);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] = val[p + q * step];
+        for (int q = 0; q < 4; ++q) r[q] = __builtin_nontemporal_load(val + p + q * step);
         if (SUB) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) r[q] = r[q] - xs[c[q]] * ysr;
@@ -56,7 +56,7 @@ __device__ __forceinline__ void strided_range(const SweepArgs& a, const IdxT* __
         }
         if (SUB || ADD) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) val[p + q * step] = r[q];
+            for (int q = 0; q < 4; ++q) __builtin_nontemporal_store(r[q], val + p + q * step);
         }
         if (SWEEP) {
 #pragma unroll
@@ -68,17 +68,79 @@ __device__ __forceinline__ void strided_range(const SweepArgs& a, const IdxT* __
         }
     }
     for (; p < end; p += step) {
-        const uint32_t c = idx[p];
-        float r = val[p];
+        const uint32_t c = __builtin_nontemporal_load(idx + p);
+        float r = __builtin_nontemporal_load(val + p);
         if (SUB) r = r - xs[c] * ysr;
         if (ADD) r = r + xa[c] * yar;
-        if (SUB || ADD) val[p] = r;
+        if (SUB || ADD) __builtin_nontemporal_store(r, val + p);
         if (SWEEP) {
             const float xv = x[c];
             g += xv * r;
             h += xv * xv;
         }
     }
+}
+
+// Four consecutive elements starting at 4*q (16-byte aligned): one float4
+// value load, one uint4 (or uint2 for 16-bit) index load.
+template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void group4(const SweepArgs& a, const IdxT* __restrict__ idx, uint32_t q, float ysr,
+                                       float yar, float& g, float& h) {
+    vf4* __restrict__ v4 = reinterpret_cast<vf4*>(a.val) + q;
+    const vf4 rv = __builtin_nontemporal_load(v4);
+    uint32_t c[4];
+    if constexpr (sizeof(IdxT) == 2) {
+        const vu2 cv = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(idx) + q);
+        c[0] = cv.x & 0xffffu;
+        c[1] = cv.x >> 16;
+        c[2] = cv.y & 0xffffu;
+        c[3] = cv.y >> 16;
+    } else {
+        const vu4 cv = __builtin_nontemporal_load(reinterpret_cast<const vu4*>(idx) + q);
+        c[0] = cv.x;
+        c[1] = cv.y;
+        c[2] = cv.z;
+        c[3] = cv.w;
+    }
+    float r[4] = {rv.x, rv.y, rv.z, rv.w};
+    if (SUB) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = r[j] - a.xs[c[j]] * ysr;
+    }
+    if (ADD) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = r[j] + a.xa[c[j]] * yar;
+    }
+    if (SUB || ADD) __builtin_nontemporal_store(vf4{r[0], r[1], r[2], r[3]}, v4);
+    if (SWEEP) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float xv = a.x[c[j]];
+            g += xv * r[j];
+            h += xv * xv;
+        }
+    }
+}
+
+// Elements [s, e) by a whole wave: an unaligned head and tail of at most
+// three elements each, then the aligned body as 16-byte groups, two groups
+// (eight elements) per lane in flight.
+template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void wave_range(const SweepArgs& a, const IdxT* __restrict__ idx, uint32_t s, uint32_t e,
+                                           uint32_t lane, float ysr, float yar, float& g, float& h) {
+    uint32_t b0 = (s + 3u) & ~3u;
+    if (b0 > e) b0 = e;
+    uint32_t b1 = e & ~3u;
+    if (b1 < b0) b1 = b0;
+    if (lane < b0 - s) strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, s + lane, s + lane + 1, 1, ysr, yar, g, h);
+    if (lane < e - b1) strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, b1 + lane, b1 + lane + 1, 1, ysr, yar, g, h);
+    const uint32_t q1 = b1 >> 2;
+    uint32_t q = (b0 >> 2) + lane;
+    for (; q + WAVE < q1; q += 2 * WAVE) {
+        group4<IdxT, SUB, ADD, SWEEP>(a, idx, q, ysr, yar, g, h);
+        group4<IdxT, SUB, ADD, SWEEP>(a, idx, q + WAVE, ysr, yar, g, h);
+    }
+    if (q < q1) group4<IdxT, SUB, ADD, SWEEP>(a, idx, q, ysr, yar, g, h);
 }
 
 template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
@@ -95,7 +157,7 @@ __global__ __launch_bounds__(BLOCK) void sweep_kernel(SweepArgs a) {
         const float ysr = SUB ? a.ys[row] : 0.0f;
         const float yar = ADD ? a.ya[row] : 0.0f;
         float g = 0.0f, h = 0.0f;
-        strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, unit.z + lane, unit.w, WAVE, ysr, yar, g, h);
+        wave_range<IdxT, SUB, ADD, SWEEP>(a, idx, unit.z, unit.w, (uint32_t)lane, ysr, yar, g, h);
         if (SWEEP) {
 #pragma unroll
             for (int off = WAVE / 2; off > 0; off >>= 1) {
@@ -121,7 +183,8 @@ __global__ __launch_bounds__(BLOCK) void sweep_kernel(SweepArgs a) {
         const float ysr = (SUB && valid) ? a.ys[row] : 0.0f;
         const float yar = (ADD && valid) ? a.ya[row] : 0.0f;
         float g = 0.0f, h = 0.0f;
-        strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, s + sl, e, G, ysr, yar, g, h);
+        if (lg == 6) wave_range<IdxT, SUB, ADD, SWEEP>(a, idx, s, e, (uint32_t)lane, ysr, yar, g, h);
+        else strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, s + sl, e, G, ysr, yar, g, h);
         if (SWEEP) {
             for (uint32_t off = G >> 1; off > 0; off >>= 1) {
                 g += __shfl_xor(g, (int)off);

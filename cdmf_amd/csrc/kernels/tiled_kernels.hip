@@ -1,0 +1,300 @@
+// Tiled (CSR5-style) sweeps for gfx950 -- the "Load Balancing" variant.
+//
+// Replaces the dual-output CSR5 SpMV of src/detail/kcode/csr5_spmv.cl
+// (spmv_csr5_compute/calibrate/tail_partition, :134-565) together with the
+// CALV/CALU divide of kcode/ccd_3.cl:1-42 and the UpdateRating kernels that
+// V3 could only run after transposing the tiles back to CSR
+// (src/cdmf_csr5.cpp:245-325).  Here the residual is updated in the tiled
+// layout itself, in the same pass as the sweep, so no re-transposition is
+// ever needed.
+//
+// One wave per tile of 64*SIGMA elements.  Lane l owns elements
+// l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of four,
+// element i of lane l at (i/4)*256 + 4*l + i%4, so one 16-byte load per lane
+// moves 1 KiB of consecutive values (512 B of 16-bit indices) per wave.  The
+// streams are read and written with non-temporal hints so that they do not
+// evict the gathered factor vector from L2.  Each lane updates its elements
+// and keeps a running (g, h) for the row it is in.  A row that starts and ends inside a
+// lane is finished by that lane.  Rows crossing lane boundaries are combined
+// by a segmented inclusive scan over the 64 lanes (shuffles, fixed order);
+// rows crossing tile boundaries leave per-tile partials (head/tail) that
+// tiled_span_kernel adds up in tile order.  No atomics: results are
+// run-to-run deterministic.  Built with -ffp-contract=off like the binned
+// kernels.
+#include "ccd_kernels.h"
+
+namespace ccd {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WPB = BLOCK / WAVE;
+
+__device__ __forceinline__ uint32_t seg_row(uint32_t seg, uint32_t row0, uint32_t segoff,
+                                            const uint32_t* __restrict__ segrows) {
+    return segoff == 0xffffffffu ? row0 + seg : segrows[segoff + seg];
+}
+
+__device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, float g, float h) {
+    const uint32_t d = a.ptr[row + 1] - a.ptr[row];
+    a.y[row] = g / (a.lambda * (float)d + h);
+}
+
+template <typename IdxT, int SIGMA, bool SUB, bool ADD, bool SWEEP>
+__global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
+    constexpr uint32_t TS = WAVE * SIGMA;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t tile = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (tile >= a.ntiles) return;
+    const IdxT* __restrict__ tidx = static_cast<const IdxT*>(a.tidx) + (size_t)tile * TS;
+    float* __restrict__ tval = a.tval + (size_t)tile * TS;
+    const uint32_t flags = a.lane_flags[(size_t)tile * WAVE + lane];
+    const uint32_t seg0 = a.lane_seg0[(size_t)tile * WAVE + lane];
+    const uint32_t row0 = a.tile_row0[tile];
+    const uint32_t segoff = a.tile_segoff[tile];
+    const uint64_t lbase = (uint64_t)tile * TS + (uint64_t)lane * SIGMA;
+
+    uint32_t c[SIGMA];
+    float r[SIGMA];
+#pragma unroll
+    for (int q = 0; q < SIGMA / 4; ++q) {
+        const vf4 rv = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(tval + q * 256 + 4 * lane));
+        r[4 * q + 0] = rv.x;
+        r[4 * q + 1] = rv.y;
+        r[4 * q + 2] = rv.z;
+        r[4 * q + 3] = rv.w;
+        if constexpr (sizeof(IdxT) == 2) {
+            const vu2 cv = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(tidx + q * 256 + 4 * lane));
+            c[4 * q + 0] = cv.x & 0xffffu;
+            c[4 * q + 1] = cv.x >> 16;
+            c[4 * q + 2] = cv.y & 0xffffu;
+            c[4 * q + 3] = cv.y >> 16;
+        } else {
+            const vu4 cv = __builtin_nontemporal_load(reinterpret_cast<const vu4*>(tidx + q * 256 + 4 * lane));
+            c[4 * q + 0] = cv.x;
+            c[4 * q + 1] = cv.y;
+            c[4 * q + 2] = cv.z;
+            c[4 * q + 3] = cv.w;
+        }
+    }
+
+    uint32_t seg = seg0;
+    uint32_t row = seg_row(seg, row0, segoff, a.segrows);
+    float ysr = SUB ? a.ys[row] : 0.0f;
+    float yar = ADD ? a.ya[row] : 0.0f;
+    float g = 0.0f, h = 0.0f, gA = 0.0f, hA = 0.0f;
+    bool has_start = (flags & 1u) != 0;
+#pragma unroll
+    for (int i = 0; i < SIGMA; ++i) {
+        if (i > 0 && ((flags >> i) & 1u)) {
+            if (SWEEP) {
+                if (!has_start) {  // the row that entered this lane ends here
+                    gA = g;
+                    hA = h;
+                } else {  // a row that started inside this lane is complete
+                    finish_row(a, row, g, h);
+                }
+            }
+            has_start = true;
+            g = 0.0f;
+            h = 0.0f;
+            ++seg;
+            row = seg_row(seg, row0, segoff, a.segrows);
+            if (SUB) ysr = a.ys[row];
+            if (ADD) yar = a.ya[row];
+        }
+        const bool valid = lbase + (uint64_t)i < a.nnz;
+        float v = r[i];
+        if (SUB) v = v - a.xs[c[i]] * ysr;
+        if (ADD) v = v + a.xa[c[i]] * yar;
+        r[i] = v;
+        if (SWEEP && valid) {
+            const float xv = a.x[c[i]];
+            g += xv * v;
+            h += xv * xv;
+        }
+    }
+    if (SUB || ADD) {
+        // padding positions of the last tile are rewritten too; they are never read as data
+#pragma unroll
+        for (int q = 0; q < SIGMA / 4; ++q)
+            __builtin_nontemporal_store(vf4{r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]},
+                                        reinterpret_cast<vf4*>(tval + q * 256 + 4 * lane));
+    }
+    if (!SWEEP) return;
+
+    // Segmented inclusive scan over lanes of the lane's contribution to the
+    // row open at its end; a lane with a row start begins a new segment.
+    float sg = g, sh = h;
+    uint32_t f = has_start ? 1u : 0u;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const float pg = __shfl_up(sg, d);
+        const float ph = __shfl_up(sh, d);
+        const uint32_t pf = __shfl_up(f, d);
+        if (lane >= d && !f) {
+            sg = pg + sg;
+            sh = ph + sh;
+            f = pf;
+        }
+    }
+    float cg = __shfl_up(sg, 1);
+    float ch = __shfl_up(sh, 1);
+    if (lane == 0) {
+        cg = 0.0f;
+        ch = 0.0f;
+    }
+    const uint32_t open = a.tile_open[tile];
+    if (has_start) {
+        const bool bit0 = (flags & 1u) != 0;
+        if (!(bit0 && lane == 0)) {
+            // the row that entered this lane closes at the lane's first row start
+            const uint32_t seg_in = bit0 ? seg0 - 1u : seg0;
+            const float tg = bit0 ? cg : cg + gA;
+            const float th = bit0 ? ch : ch + hA;
+            if (seg_in == 0 && (open & 1u)) a.head[tile] = make_float2(tg, th);
+            else finish_row(a, seg_row(seg_in, row0, segoff, a.segrows), tg, th);
+        }
+    }
+    if (lane == WAVE - 1) {
+        // the row open at the end of the tile
+        if (seg == 0 && (open & 1u)) a.head[tile] = make_float2(sg, sh);
+        else if (open & 2u) a.tail[tile] = make_float2(sg, sh);
+        else finish_row(a, row, sg, sh);
+    }
+}
+
+// Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], one wave
+// per row, lane-strided partial sums reduced in a fixed tree.
+__global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t i = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (i >= a.nspan) return;
+    const uint32_t row = a.span_rows[i];
+    const uint32_t s = a.ptr[row], e = a.ptr[row + 1];
+    const uint32_t t0 = s / a.tile_elems, t1 = (e - 1) / a.tile_elems;
+    float g = 0.0f, h = 0.0f;
+    for (uint32_t t = t0 + 1 + lane; t <= t1; t += WAVE) {
+        const float2 p = a.head[t];
+        g += p.x;
+        h += p.y;
+    }
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        g += __shfl_xor(g, off);
+        h += __shfl_xor(h, off);
+    }
+    if (lane == 0) {
+        const float2 p = a.tail[t0];
+        g = p.x + g;
+        h = p.y + h;
+        a.y[row] = g / (a.lambda * (float)(e - s) + h);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void retile_f32_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                           uint64_t nnz, uint64_t padded, uint32_t sigma,
+                                                           bool to_tiled) {
+    const uint64_t ts = (uint64_t)WAVE * sigma;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
+        const uint64_t tile = e / ts, rem = e % ts;
+        const uint64_t i = rem % sigma, l = rem / sigma;
+        const uint64_t pos = tile * ts + (i >> 2) * (4 * WAVE) + 4 * l + (i & 3);
+        if (to_tiled) dst[pos] = e < nnz ? src[e] : 0.0f;
+        else if (e < nnz) dst[e] = src[pos];
+    }
+}
+
+template <typename IdxT>
+__global__ __launch_bounds__(BLOCK) void retile_idx_kernel(const IdxT* __restrict__ src, IdxT* __restrict__ dst,
+                                                           uint64_t nnz, uint64_t padded, uint32_t sigma) {
+    const uint64_t ts = (uint64_t)WAVE * sigma;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
+        const uint64_t tile = e / ts, rem = e % ts;
+        const uint64_t i = rem % sigma, l = rem / sigma;
+        const uint64_t pos = tile * ts + (i >> 2) * (4 * WAVE) + 4 * l + (i & 3);
+        dst[pos] = e < nnz ? src[e] : (IdxT)0;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void fill_kernel(float* __restrict__ dst, float value, uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < count; e += stride) dst[e] = value;
+}
+
+uint32_t grid_for(uint64_t count) {
+    uint64_t b = (count + BLOCK - 1) / BLOCK;
+    if (b > 16384) b = 16384;
+    return (uint32_t)(b ? b : 1);
+}
+
+template <typename IdxT, int SIGMA>
+hipError_t launch_tiled_s(const TiledArgs& a, bool sub, bool add, bool sweep, hipStream_t s) {
+    const uint32_t blocks = (a.ntiles + WPB - 1) / WPB;
+    if (blocks == 0) return hipSuccess;
+    const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
+    switch (code) {
+        case 1: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, false, false, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, false, true, false>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, false, true, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, false, false>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, false, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, true, false>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, true, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        default: return hipSuccess;
+    }
+    return hipGetLastError();
+}
+
+template <typename IdxT>
+hipError_t launch_tiled_i(const TiledArgs& a, int sigma, bool sub, bool add, bool sweep, hipStream_t s) {
+    switch (sigma) {
+        case 8: return launch_tiled_s<IdxT, 8>(a, sub, add, sweep, s);
+        case 16: return launch_tiled_s<IdxT, 16>(a, sub, add, sweep, s);
+        case 32: return launch_tiled_s<IdxT, 32>(a, sub, add, sweep, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_tiled(const TiledArgs& a, int sigma, bool idx16, bool sub, bool add, bool sweep, hipStream_t s) {
+    return idx16 ? launch_tiled_i<uint16_t>(a, sigma, sub, add, sweep, s)
+                 : launch_tiled_i<uint32_t>(a, sigma, sub, add, sweep, s);
+}
+
+hipError_t launch_span(const SpanArgs& a, hipStream_t s) {
+    if (a.nspan == 0) return hipSuccess;
+    hipLaunchKernelGGL(tiled_span_kernel, dim3((a.nspan + WPB - 1) / WPB), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_t ntiles, int sigma, bool to_tiled,
+                             hipStream_t s) {
+    const uint64_t padded = (uint64_t)ntiles * WAVE * (uint64_t)sigma;
+    hipLaunchKernelGGL(retile_f32_kernel, dim3(grid_for(padded)), dim3(BLOCK), 0, s, src, dst, nnz, padded,
+                       (uint32_t)sigma, to_tiled);
+    return hipGetLastError();
+}
+
+hipError_t launch_retile_idx(const void* src, void* dst, bool idx16, uint64_t nnz, uint32_t ntiles, int sigma,
+                             hipStream_t s) {
+    const uint64_t padded = (uint64_t)ntiles * WAVE * (uint64_t)sigma;
+    if (idx16)
+        hipLaunchKernelGGL(retile_idx_kernel<uint16_t>, dim3(grid_for(padded)), dim3(BLOCK), 0, s,
+                           static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), nnz, padded, (uint32_t)sigma);
+    else
+        hipLaunchKernelGGL(retile_idx_kernel<uint32_t>, dim3(grid_for(padded)), dim3(BLOCK), 0, s,
+                           static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), nnz, padded, (uint32_t)sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float* dst, float value, uint64_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(count)), dim3(BLOCK), 0, s, dst, value, count);
+    return hipGetLastError();
+}
+
+}  // namespace ccd

@@ -119,6 +119,11 @@ int32_t ccd_upload_test(ccd_ctx* ctx, uint64_t nnz, const uint32_t* row, const u
  * start state, valid when W H^T = 0). */
 int32_t ccd_set_factors(ccd_ctx* ctx, uint32_t k, const float* W, const float* H);
 
+/* Runs maxiter outer iterations.  The first run after ccd_set_factors starts
+ * from the residual R (the reference's start: W H^T = 0, skipping the
+ * add-back in outer iteration 1).  A later run continues the same solve: the
+ * residual then holds R - W H^T and every rank is added back first; such a
+ * run must have zero_first = 0. */
 int32_t ccd_run(ccd_ctx* ctx, const ccd_params* params, ccd_iter_cb cb, void* user);
 
 int32_t ccd_get_factors(ccd_ctx* ctx, float* W, float* H);
@@ -162,13 +167,15 @@ int32_t ccd_loop_time(ccd_ctx* ctx, double* seconds);
  * scale rocprofv3 FETCH_SIZE/WRITE_SIZE counters. */
 int32_t ccd_copy_probe(ccd_ctx* ctx, uint64_t bytes, int32_t reps, double* ms_per_copy);
 
-/* Work-plan knobs of the binned sweeps; they only change speed, never results'
- * tolerance class.  Applied at the next ccd_upload. */
+/* Work-plan knobs of the binned and tiled sweeps; they only change speed and
+ * the order of floating-point sums.  The binned knobs apply at the next
+ * ccd_upload, sigma at the next run of the tiled variant. */
 typedef struct ccd_plan_params {
     uint32_t elems_per_lane;  /* target elements per lane when sizing sub-wave groups (default 4) */
     uint32_t split_threshold; /* rows longer than this are split across waves (default 8192) */
     uint32_t piece;           /* elements per piece of a split row (default 4096) */
     uint32_t unit_elems;      /* target elements per wave work unit (default 4096) */
+    uint32_t sigma;           /* elements per lane of a tile of the tiled sweeps: 8, 16 or 32 (default 16) */
 } ccd_plan_params;
 
 int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);

@@ -52,6 +52,20 @@ def _close(x, ref, what):
     return frob
 
 
+def _close_bulk(x, ref, what, quantile=0.999, max_abs=1e-2):
+    """Full-size FP32 bar: Frobenius relative 1e-3, at least `quantile` of the
+    entries inside the elementwise bound of _close, none off by more than
+    max_abs."""
+    x = np.asarray(x, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    frob = np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300)
+    d = np.abs(x - ref)
+    inside = np.mean(d <= F_RTOL * np.abs(ref) + F_RTOL * _rms(ref))
+    _say("%s: Frobenius %.3g, inside elementwise bound %.6f, max |d| %.3g" % (what, frob, inside, d.max()))
+    assert frob <= F_RTOL and inside >= quantile and d.max() <= max_abs, what
+    return frob
+
+
 def _compare(W, H, rm, rcsr, rcsc, Wo, Ho, rmo, rcsro, rcsco, label):
     drm = np.max(np.abs(np.array(rm) - rmo)) if len(rm) else 0.0
     _say("[%s] max|dRMSE|=%.3g  max|dW|=%.3g  max|dH|=%.3g  max|dres|=%.3g" % (
@@ -184,17 +198,39 @@ def test_residual_is_r_minus_wht(oracle_lib):
 @pytest.mark.skipif(os.environ.get("CDMF_SKIP_LARGE") == "1", reason="large case disabled by CDMF_SKIP_LARGE=1")
 def test_netflix_shape_bounded(oracle_lib):
     """Full 480,189 x 17,770 / 99.1M-nnz matrix, two ranks of two outer
-    iterations, against the oracle on the host cores."""
+    iterations.
+
+    At this size the reference's serial FP32 sums over columns of up to
+    ~400k entries drift from the exact result: the FP32 oracle differs from
+    the FP64 oracle by ~5 % (Frobenius) in W, through users with one or two
+    ratings whose update amplifies an item-factor error by up to r/lambda.
+    So W, H and the residual are held to the 1e-3 bar against the FP64
+    oracle -- Frobenius relative 1e-3, 99.9 % of entries inside the
+    elementwise bound and none off by more than 1e-2 (the few hundred users
+    with two ratings amplify FP32 rounding of the item factors) -- the
+    per-iteration test RMSE to 1e-4 against the FP32 oracle, and the GPU
+    must be closer to FP64 than the FP32 oracle is."""
     from cdmf_amd import data as D
     _say("case netflix-shape k=2 outer=2 inner=2: generating")
     ds = D.netflix()
     k, lam = 2, 0.05
     W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
     _say("running gpu")
-    got = _run_gpu(ds, W0, H0, k, lam, 2, 2)
-    _say("running oracle")
-    ref = oracle_lib.ccd(ds, W0, H0, k, lam, 2, 2, threads=min(16, os.cpu_count() or 1))
-    _compare(*got, *ref, "netflix k=2")
+    W, H, rm, rcsr, rcsc = _run_gpu(ds, W0, H0, k, lam, 2, 2)
+    threads = min(16, os.cpu_count() or 1)
+    _say("running FP32 oracle")
+    r32 = oracle_lib.ccd(ds, W0, H0, k, lam, 2, 2, threads=threads)
+    np.testing.assert_allclose(rm, r32[2], rtol=0, atol=RMSE_ATOL)
+    _say("running FP64 oracle")
+    W64, H64, rm64, res64, _ = oracle_lib.ccd(ds, W0, H0, k, lam, 2, 2, threads=threads, dtype="f64")
+    _say("[netflix k=2] |dRMSE| vs fp32 %.3g vs fp64 %.3g" % (np.max(np.abs(np.array(rm) - r32[2])),
+                                                             np.max(np.abs(np.array(rm) - rm64))))
+    fw = _close_bulk(W, W64, "W vs FP64 oracle")
+    fh = _close_bulk(H, H64, "H vs FP64 oracle")
+    fr = _close_bulk(rcsr, res64, "residual vs FP64 oracle")
+    f32w = np.linalg.norm(r32[0] - W64) / np.linalg.norm(W64)
+    _say("[netflix k=2] Frobenius vs FP64: gpu W %.3g H %.3g res %.3g; fp32 oracle W %.3g" % (fw, fh, fr, f32w))
+    assert fw < f32w
 
 
 def test_gpu_within_oracle_noise(oracle_lib):
@@ -218,3 +254,60 @@ def test_gpu_within_oracle_noise(oracle_lib):
         d_twin = np.linalg.norm(twin[q] - ref[q])
         _say("%s: ||gpu - oracle|| = %.3g, ||reversed-order oracle - oracle|| = %.3g" % (name, d_gpu, d_twin))
         assert d_gpu <= 2.0 * d_twin
+
+
+# ------------------------------------------------------------ tiled sweeps
+
+def _holey(ds):
+    """The small matrix with every 3rd row and every 5th column emptied in
+    training: empty rows inside tiles exercise the segment table."""
+    from cdmf_amd import datadir
+    rows = np.repeat(np.arange(ds.m), np.diff(ds.csr_ptr.astype(np.int64)))
+    keep = (rows % 3 != 0) & (ds.csr_idx % 5 != 0)
+    return datadir.from_triplets(ds.m, ds.n, (rows[keep], ds.csr_idx[keep], ds.csr_val[keep]),
+                                 (ds.test_row, ds.test_col, ds.test_val))
+
+
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_tiled_variant(oracle_lib, sigma):
+    from cdmf_amd import data as D
+    cases = []
+    for name in ("simple", "tricky_simple"):
+        g = load_golden(name)
+        cases.append((name, golden_dataset(g), int(g["k"]), float(g["lam"]), int(g["outer"]), int(g["inner"])))
+    small = _small()
+    cases.append(("small", small, 10, 0.05, 3, 3))
+    cases.append(("holey", _holey(small), 6, 0.05, 3, 2))
+    cases.append(("powerlaw", D.gen_powerlaw(m=3000, n=800, nnz_train=120000, nnz_test=6000, max_row_degree=800,
+                                             rating_kind=1, seed=7), 6, 0.05, 3, 3))
+    for name, ds, k, lam, outer, inner in cases:
+        _say("case tiled sigma=%d %s" % (sigma, name))
+        W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+        got = _run_gpu(ds, W0, H0, k, lam, outer, inner, variant=3, plan=dict(sigma=sigma))
+        ref = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner)
+        _compare(*got, *ref, "tiled s=%d %s" % (sigma, name))
+        empty = np.flatnonzero(np.diff(ds.csc_ptr.astype(np.int64)) == 0)
+        assert np.all(got[1][:, empty] == 0.0)  # empty columns -> 0 (the reference's V3 would give NaN)
+
+
+def test_tiled_wfirst_and_switching(oracle_lib):
+    """W-first order on the tiled path, then a binned run continuing from the
+    tiled residual: the residual hand-over between layouts is exact."""
+    from cdmf_amd import data as D
+    solver = _solver()
+    ds = _small()
+    c = D.SMALL_SOLVE
+    W0, H0 = D.small_init(ds, c.k)
+    ref = oracle_lib.ccd(ds, W0, H0, c.k, c.lam, c.outer, c.inner, order=1)
+    with solver.Solver(0) as s:
+        s.upload(ds)
+        s.set_factors(W0, H0)
+        s.run(c.k, c.lam, 2, c.inner, variant=3, order=1)
+        W2, H2 = s.factors()
+        r_tiled = s.residual()
+        res = s.run(c.k, c.lam, c.outer - 2, c.inner, variant=1, order=1, zero_first=False)
+        W, H = s.factors()
+        rcsr, rcsc = s.residual()
+    ref2 = oracle_lib.ccd(ds, W0, H0, c.k, c.lam, 2, c.inner, order=1)
+    _compare(W2, H2, [], r_tiled[0], r_tiled[1], *ref2[:2], [], ref2[3], ref2[4], "tiled w-first 2 outer")
+    _compare(W, H, res.rmse, rcsr, rcsc, ref[0], ref[1], ref[2][2:], ref[3], ref[4], "then binned 3 outer")
