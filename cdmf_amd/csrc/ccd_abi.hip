@@ -14,6 +14,15 @@
 // (val - u_{t-1} v_{t-1}) + w_t h_t, each product and sum rounded to FP32.
 // After the last rank the pending subtraction is applied on its own, so the
 // residual left on the device equals the one the reference leaves in R.
+//
+// Internal numbering: rows and columns are renumbered by falling degree at
+// upload (stable, so ties keep their order).  The highest-degree rows and
+// columns -- the entries gathered most often -- become a contiguous prefix
+// of every factor vector, which the sweeps stage in LDS, and the sweep work
+// units become contiguous row ranges.  Element order inside a row or column
+// is kept, so every per-row sum adds the same terms in the same order as
+// without renumbering.  W, H, the test set and the residual are mapped at
+// the boundary; callers only ever see their own numbering.
 #include "cdmf_hip.h"
 #include "ccd_kernels.h"
 
@@ -40,8 +49,8 @@ struct TiledLayout {
     uint32_t* segoff = nullptr;
     uint32_t* segrows = nullptr;
     uint8_t* open = nullptr;
-    float2* head = nullptr;
-    float2* tail = nullptr;
+    double2* head = nullptr;  // (g, h) pairs; double-sized so every precision fits
+    double2* tail = nullptr;
     uint32_t* span_rows = nullptr;
     uint32_t nspan = 0;
 };
@@ -57,13 +66,14 @@ struct DevLayout {
     float* val0 = nullptr;  // R itself, to reset the residual
     uint4* units = nullptr;
     uint32_t nunits = 0;
-    uint32_t* rowlist = nullptr;
     uint32_t* split_rows = nullptr;
     uint32_t* split_beg = nullptr;
     uint32_t nsplit = 0;
-    float2* partials = nullptr;
+    double2* partials = nullptr;  // (g, h) per split piece, double-sized for every precision
     uint32_t npieces = 0;
-    std::vector<uint32_t> hptr;  // host copy of ptr (tile planning)
+    std::vector<uint32_t> hptr;      // host copy of the internal ptr (planning)
+    std::vector<uint32_t> ext_ptr;   // the caller's ptr (residual download)
+    std::vector<uint32_t> perm;      // internal row -> caller's row
     TiledLayout tl;
 };
 
@@ -94,6 +104,15 @@ struct ccd_ctx {
     float* H = nullptr;
     float* u = nullptr;
     float* v = nullptr;
+    // bfloat16 copies of W, H, u, v for runs with CCD_PREC_BF16
+    uint16_t* W16 = nullptr;
+    uint16_t* H16 = nullptr;
+    uint16_t* u16 = nullptr;
+    uint16_t* v16 = nullptr;
+    uint32_t k16 = 0;
+    int prec = CCD_PREC_FP32;  // precision of the run in progress
+    uint32_t sweep_blocks = 0; // persistent grid: 2 blocks of 1024 threads per CU
+    size_t lds_budget = 0;     // LDS bytes per block for the staged hot prefix
     double* rmse_bsum = nullptr;
     double* rmse_dev = nullptr;  // one slot per outer iteration
     int32_t rmse_cap = 0;
@@ -156,7 +175,6 @@ void free_layout(DevLayout& L) {
     dfree(L.val);
     dfree(L.val0);
     dfree(L.units);
-    dfree(L.rowlist);
     dfree(L.split_rows);
     dfree(L.split_beg);
     dfree(L.partials);
@@ -166,7 +184,6 @@ void free_layout(DevLayout& L) {
 // ----------------------------------------------------------- planning ---
 struct HostPlan {
     std::vector<uint4> units;
-    std::vector<uint32_t> rowlist;
     std::vector<uint32_t> split_rows, split_beg;
 };
 
@@ -178,40 +195,33 @@ inline int group_log2(uint32_t d, uint32_t epl) {
 }
 
 HostPlan make_plan(const uint32_t* ptr, int64_t rows, const ccd_plan_params& pp) {
+    // rows arrive sorted by falling degree: split rows first, then groups of
+    // equal width as contiguous ranges
     HostPlan P;
-    std::vector<uint32_t> order, split;
-    order.reserve(rows);
-    for (int64_t r = 0; r < rows; ++r) {
-        const uint32_t d = ptr[r + 1] - ptr[r];
-        (d > pp.split_threshold ? split : order).push_back((uint32_t)r);
-    }
-    auto deg = [&](uint32_t r) { return ptr[r + 1] - ptr[r]; };
-    std::stable_sort(split.begin(), split.end(), [&](uint32_t a, uint32_t b) { return deg(a) > deg(b); });
+    auto deg = [&](int64_t r) { return ptr[r + 1] - ptr[r]; };
     uint32_t slot = 0;
-    for (uint32_t r : split) {
-        P.split_rows.push_back(r);
+    int64_t r = 0;
+    for (; r < rows && deg(r) > pp.split_threshold; ++r) {
+        P.split_rows.push_back((uint32_t)r);
         P.split_beg.push_back(slot);
         for (uint32_t s = ptr[r]; s < ptr[r + 1]; s += pp.piece) {
             const uint32_t e = std::min<uint64_t>((uint64_t)s + pp.piece, ptr[r + 1]);
-            P.units.push_back(make_uint4(ccd::UNIT_SPLIT | (slot << 8), r, s, e));
+            P.units.push_back(make_uint4(ccd::UNIT_SPLIT | (slot << 8), (uint32_t)r, s, e));
             ++slot;
         }
     }
     P.split_beg.push_back(slot);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return deg(a) > deg(b); });
-    P.rowlist = order;
-    const size_t N = order.size();
     const uint32_t max_rows = 4096;
-    size_t i = 0;
-    while (i < N) {
-        const int lg = group_log2(deg(order[i]), pp.elems_per_lane);
+    int64_t i = r;
+    while (i < rows) {
+        const int lg = group_log2(deg(i), pp.elems_per_lane);
         const uint32_t rpp = 64u >> lg;
-        size_t j = i;
+        int64_t j = i;
         uint64_t elems = 0;
-        while (j < N && group_log2(deg(order[j]), pp.elems_per_lane) == lg) {
-            const size_t cnt = j - i;
+        while (j < rows && group_log2(deg(j), pp.elems_per_lane) == lg) {
+            const uint64_t cnt = (uint64_t)(j - i);
             if (cnt % rpp == 0 && (elems >= pp.unit_elems || cnt >= max_rows)) break;
-            elems += deg(order[j]);
+            elems += deg(j);
             ++j;
         }
         P.units.push_back(make_uint4(ccd::UNIT_GROUP | ((uint32_t)lg << 8), (uint32_t)i, (uint32_t)j, 0u));
@@ -269,23 +279,34 @@ struct RunState {
     const ccd_params* p;
 };
 
-int do_pass(ccd_ctx* c, DevLayout& L, const float* x, float* y, const float* xs, const float* ys,
-            const float* xa, const float* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
-    ccd::SweepArgs a{L.ptr, L.idx, L.val, x, y, xs, ys, xa, ya, L.units, L.rowlist, L.partials, L.nunits, lambda};
+// Entries of the gathered vector staged in LDS: as many of the leading
+// (highest-degree) ones as the per-block LDS budget holds.
+uint32_t hot_count(const ccd_ctx* c, const DevLayout& L, const void* x) {
+    if (!x) return 0;
+    const size_t es = c->prec == CCD_PREC_BF16 ? 2 : 4;
+    const size_t cap = c->lds_budget / es;
+    return (uint32_t)std::min<size_t>(cap, (size_t)L.gdim);
+}
+
+int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
+            const void* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
+    const void* xg = sweep ? x : nullptr;
+    ccd::SweepArgs a{L.ptr, L.idx, L.val, xg, y, xs, ys, xa, ya, L.units, L.partials, L.nunits,
+                     hot_count(c, L, xg), (uint32_t)(xg && xa == xg), lambda};
     {
         Timed tm(c, kind);
-        HIP_TRY(c, ccd::launch_sweep(a, L.idx16, sub, add, sweep, c->stream));
+        HIP_TRY(c, ccd::launch_sweep(a, c->prec, L.idx16, sub, add, sweep, c->sweep_blocks, c->stream));
     }
     if (sweep && L.nsplit) {
         Timed tm(c, CCD_K_FINALIZE);
         ccd::FinalizeArgs f{L.split_rows, L.split_beg, L.partials, L.ptr, y, lambda, L.nsplit};
-        HIP_TRY(c, ccd::launch_finalize(f, c->stream));
+        HIP_TRY(c, ccd::launch_finalize(f, c->prec, c->stream));
     }
     return CCD_OK;
 }
 
-using PassFn = int (*)(ccd_ctx*, DevLayout&, const float*, float*, const float*, const float*, const float*,
-                      const float*, bool, bool, bool, float, int);
+using PassFn = int (*)(ccd_ctx*, DevLayout&, const void*, void*, const void*, const void*, const void*,
+                      const void*, bool, bool, bool, float, int);
 
 int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t nnz, const uint32_t* ptr,
                   const uint32_t* idx, const float* val) {
@@ -321,10 +342,8 @@ int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t
     L.nsplit = (uint32_t)P.split_rows.size();
     L.npieces = P.split_beg.back();
     HIP_TRY(c, dalloc(&L.units, P.units.size()));
-    HIP_TRY(c, hipMemcpy(L.units, P.units.data(), sizeof(uint4) * P.units.size(), hipMemcpyHostToDevice));
-    HIP_TRY(c, dalloc(&L.rowlist, P.rowlist.size()));
-    if (!P.rowlist.empty())
-        HIP_TRY(c, hipMemcpy(L.rowlist, P.rowlist.data(), sizeof(uint32_t) * P.rowlist.size(), hipMemcpyHostToDevice));
+    if (!P.units.empty())
+        HIP_TRY(c, hipMemcpy(L.units, P.units.data(), sizeof(uint4) * P.units.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, dalloc(&L.split_rows, P.split_rows.size()));
     if (!P.split_rows.empty())
         HIP_TRY(c, hipMemcpy(L.split_rows, P.split_rows.data(), sizeof(uint32_t) * P.split_rows.size(),
@@ -418,21 +437,52 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
     return CCD_OK;
 }
 
-int do_tiled_pass(ccd_ctx* c, DevLayout& L, const float* x, float* y, const float* xs, const float* ys,
-                  const float* xa, const float* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
+int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
+                  const void* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
     TiledLayout& T = L.tl;
-    ccd::TiledArgs a{L.ptr, T.tidx, T.tval, x, y, xs, ys, xa, ya, T.flags, T.seg0, T.row0, T.segoff, T.segrows,
-                     T.open, T.head, T.tail, L.nnz, T.ntiles, lambda};
+    const void* xg = sweep ? x : nullptr;
+    ccd::TiledArgs a{L.ptr, T.tidx, T.tval, xg, y, xs, ys, xa, ya, T.flags, T.seg0, T.row0, T.segoff, T.segrows,
+                     T.open, T.head, T.tail, L.nnz, T.ntiles, hot_count(c, L, xg), (uint32_t)(xg && xa == xg),
+                     lambda};
     {
         Timed tm(c, kind);
-        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, L.idx16, sub, add, sweep, c->stream));
+        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, c->sweep_blocks, c->stream));
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
         ccd::SpanArgs sa{T.span_rows, L.ptr, T.head, T.tail, y, T.nspan, 64u * (uint32_t)T.sigma, lambda};
-        HIP_TRY(c, ccd::launch_span(sa, c->stream));
+        HIP_TRY(c, ccd::launch_span(sa, c->prec, c->stream));
     }
     return CCD_OK;
+}
+
+// Positions sorted by falling degree (stable).
+std::vector<uint32_t> degree_order(const uint32_t* ptr, int64_t rows) {
+    std::vector<uint32_t> perm((size_t)rows);
+    for (int64_t r = 0; r < rows; ++r) perm[r] = (uint32_t)r;
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint32_t a, uint32_t b) { return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b]; });
+    return perm;
+}
+
+// One compressed layout in internal numbering: rows in `perm` order, indices
+// mapped through `ipos` (caller's index -> internal), element order inside
+// each row kept.
+void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
+              const std::vector<uint32_t>& perm, const std::vector<uint32_t>& ipos, std::vector<uint32_t>& optr,
+              std::vector<uint32_t>& oidx, std::vector<float>& oval) {
+    optr.assign((size_t)rows + 1, 0u);
+    for (int64_t i = 0; i < rows; ++i) optr[i + 1] = optr[i] + (ptr[perm[i] + 1] - ptr[perm[i]]);
+    oidx.resize(optr[rows]);
+    oval.resize(optr[rows]);
+    for (int64_t i = 0; i < rows; ++i) {
+        const uint32_t r = perm[i];
+        uint32_t o = optr[i];
+        for (uint32_t e = ptr[r]; e < ptr[r + 1]; ++e, ++o) {
+            oidx[o] = ipos[idx[e]];
+            oval[o] = val[e];
+        }
+    }
 }
 
 const char* check_compressed(int64_t rows, int64_t dim, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
@@ -486,6 +536,16 @@ int32_t ccd_create(int32_t device, ccd_ctx** out) {
         delete c;
         return CCD_ERR_HIP;
     }
+    int ncu = 0, lds_block = 0, lds_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0) ncu = 256;
+    if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        lds_block = 65536;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        lds_cu <= 0)
+        lds_cu = lds_block;
+    c->sweep_blocks = 2u * (uint32_t)ncu;
+    // two persistent blocks per CU share its LDS; keep 1 KiB per block spare
+    c->lds_budget = (size_t)std::max(0, std::min(lds_block, lds_cu / 2) - 1024) & ~(size_t)255;
     if (dalloc(&c->rmse_bsum, ccd::RMSE_BLOCKS) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -508,6 +568,10 @@ int32_t ccd_destroy(ccd_ctx* c) {
     dfree(c->H);
     dfree(c->u);
     dfree(c->v);
+    dfree(c->W16);
+    dfree(c->H16);
+    dfree(c->u16);
+    dfree(c->v16);
     dfree(c->rmse_bsum);
     dfree(c->rmse_dev);
     for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -546,10 +610,24 @@ int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_
     c->m = m;
     c->n = n;
     c->nnz = nnz;
-    int rc = upload_layout(c, c->csr, m, n, nnz, row_ptr, col_idx, csr_val);
-    if (rc) return rc;
-    rc = upload_layout(c, c->csc, n, m, nnz, col_ptr, row_idx, csc_val);
-    if (rc) return rc;
+    {
+        std::vector<uint32_t> rperm = degree_order(row_ptr, m), cperm = degree_order(col_ptr, n);
+        std::vector<uint32_t> rpos((size_t)m), cpos((size_t)n);
+        for (int64_t i = 0; i < m; ++i) rpos[rperm[i]] = (uint32_t)i;
+        for (int64_t j = 0; j < n; ++j) cpos[cperm[j]] = (uint32_t)j;
+        std::vector<uint32_t> p2, i2;
+        std::vector<float> v2;
+        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2);
+        int rc = upload_layout(c, c->csr, m, n, nnz, p2.data(), i2.data(), v2.data());
+        if (rc) return rc;
+        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2);
+        rc = upload_layout(c, c->csc, n, m, nnz, p2.data(), i2.data(), v2.data());
+        if (rc) return rc;
+        c->csr.ext_ptr.assign(row_ptr, row_ptr + m + 1);
+        c->csc.ext_ptr.assign(col_ptr, col_ptr + n + 1);
+        c->csr.perm = std::move(rperm);
+        c->csc.perm = std::move(cperm);
+    }
     dfree(c->u);
     dfree(c->v);
     HIP_TRY(c, dalloc(&c->u, m));
@@ -575,8 +653,16 @@ int32_t ccd_upload_test(ccd_ctx* c, uint64_t nnz, const uint32_t* row, const uin
     HIP_TRY(c, dalloc(&c->tcol, nnz));
     HIP_TRY(c, dalloc(&c->tval, nnz));
     if (nnz) {
-        HIP_TRY(c, hipMemcpy(c->trow, row, 4 * nnz, hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(c->tcol, col, 4 * nnz, hipMemcpyHostToDevice));
+        // internal numbering of the test entries (order kept)
+        std::vector<uint32_t> rpos((size_t)c->m), cpos((size_t)c->n), r2(nnz), c2(nnz);
+        for (int64_t i = 0; i < c->m; ++i) rpos[c->csr.perm[i]] = (uint32_t)i;
+        for (int64_t j = 0; j < c->n; ++j) cpos[c->csc.perm[j]] = (uint32_t)j;
+        for (uint64_t e = 0; e < nnz; ++e) {
+            r2[e] = rpos[row[e]];
+            c2[e] = cpos[col[e]];
+        }
+        HIP_TRY(c, hipMemcpy(c->trow, r2.data(), 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->tcol, c2.data(), 4 * nnz, hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->tval, val, 4 * nnz, hipMemcpyHostToDevice));
     }
     return CCD_OK;
@@ -596,8 +682,15 @@ int32_t ccd_set_factors(ccd_ctx* c, uint32_t k, const float* W, const float* H) 
         HIP_TRY(c, dalloc(&c->H, (size_t)k * c->n));
         c->k = k;
     }
-    HIP_TRY(c, hipMemcpy(c->W, W, sizeof(float) * (size_t)k * c->m, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->H, H, sizeof(float) * (size_t)k * c->n, hipMemcpyHostToDevice));
+    {
+        std::vector<float> t2((size_t)k * std::max(c->m, c->n));
+        for (uint32_t t = 0; t < k; ++t)
+            for (int64_t i = 0; i < c->m; ++i) t2[(size_t)t * c->m + i] = W[(size_t)t * c->m + c->csr.perm[i]];
+        HIP_TRY(c, hipMemcpy(c->W, t2.data(), sizeof(float) * (size_t)k * c->m, hipMemcpyHostToDevice));
+        for (uint32_t t = 0; t < k; ++t)
+            for (int64_t j = 0; j < c->n; ++j) t2[(size_t)t * c->n + j] = H[(size_t)t * c->n + c->csc.perm[j]];
+        HIP_TRY(c, hipMemcpy(c->H, t2.data(), sizeof(float) * (size_t)k * c->n, hipMemcpyHostToDevice));
+    }
     HIP_TRY(c, hipMemcpy(c->csr.val, c->csr.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
     HIP_TRY(c, hipMemcpy(c->csc.val, c->csc.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
     c->factors = true;
@@ -613,21 +706,49 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         return set_err(c, CCD_ERR_INVALID, "maxiter, maxinneriter >= 0 and lambda >= 0 finite");
     if (p->variant != CCD_VARIANT_NATIVE && p->variant != CCD_VARIANT_BATCHING && p->variant != CCD_VARIANT_TILED)
         return set_err(c, CCD_ERR_INVALID, "variant must be 1, 2 or 3");
-    if (p->precision != CCD_PREC_FP32) return set_err(c, CCD_ERR_UNSUPPORTED, "precision not available in this build");
+    if (p->precision != CCD_PREC_FP32 && p->precision != CCD_PREC_BF16 && p->precision != CCD_PREC_FP64_SUMS)
+        return set_err(c, CCD_ERR_INVALID, "precision must be CCD_PREC_FP32, CCD_PREC_BF16 or CCD_PREC_FP64_SUMS");
+    if (p->variant == CCD_VARIANT_TILED && p->precision != CCD_PREC_FP32 && c->plan.sigma != 16)
+        return set_err(c, CCD_ERR_UNSUPPORTED, "tiled sweeps in BF16 or FP64-sums precision need sigma = 16");
     if (p->order != CCD_ORDER_H_FIRST && p->order != CCD_ORDER_W_FIRST) return set_err(c, CCD_ERR_INVALID, "bad order");
     if (p->zero_first && c->continued)
         return set_err(c, CCD_ERR_INVALID, "zero_first on a run that continues a previous one (call ccd_set_factors first)");
     HIP_TRY(c, hipSetDevice(c->device));
 
     const bool hfirst = p->order == CCD_ORDER_H_FIRST;
+    const bool bf = p->precision == CCD_PREC_BF16;
+    c->prec = p->precision;
     DevLayout& F = hfirst ? c->csc : c->csr;
     DevLayout& S = hfirst ? c->csr : c->csc;
-    float* XF = hfirst ? c->H : c->W;
-    float* XS = hfirst ? c->W : c->H;
-    float* vF = hfirst ? c->v : c->u;  // output of the first sweep
-    float* vS = hfirst ? c->u : c->v;
     const int64_t dF = F.rows, dS = S.rows;
     const uint32_t k = c->k;
+    // FP32 master factors; the zeroing of the first-swept one happens there
+    if (p->zero_first)
+        HIP_TRY(c, hipMemsetAsync(hfirst ? c->H : c->W, 0, sizeof(float) * (size_t)k * dF, c->stream));
+    if (bf) {
+        if (c->k16 != k) {
+            dfree(c->W16);
+            dfree(c->H16);
+            c->W16 = c->H16 = nullptr;
+            HIP_TRY(c, dalloc(&c->W16, (size_t)k * c->m));
+            HIP_TRY(c, dalloc(&c->H16, (size_t)k * c->n));
+            c->k16 = k;
+        }
+        if (!c->u16) HIP_TRY(c, dalloc(&c->u16, c->m));
+        if (!c->v16) HIP_TRY(c, dalloc(&c->v16, c->n));
+        HIP_TRY(c, ccd::launch_to_bf16(c->W, c->W16, (uint64_t)k * c->m, c->stream));
+        HIP_TRY(c, ccd::launch_to_bf16(c->H, c->H16, (uint64_t)k * c->n, c->stream));
+    }
+    // factor storage of this run: byte pointers, element size es
+    const size_t es = bf ? 2 : 4;
+    char* const Wr = bf ? reinterpret_cast<char*>(c->W16) : reinterpret_cast<char*>(c->W);
+    char* const Hr = bf ? reinterpret_cast<char*>(c->H16) : reinterpret_cast<char*>(c->H);
+    char* const ur = bf ? reinterpret_cast<char*>(c->u16) : reinterpret_cast<char*>(c->u);
+    char* const vr = bf ? reinterpret_cast<char*>(c->v16) : reinterpret_cast<char*>(c->v);
+    char* const XF = hfirst ? Hr : Wr;
+    char* const XS = hfirst ? Wr : Hr;
+    char* const vF = hfirst ? vr : ur;  // output of the first sweep
+    char* const vS = hfirst ? ur : vr;
     const float lambda = p->lambda;
     const int T = p->maxiter, Tin = p->maxinneriter;
     const bool do_rmse = p->rmse_every && c->nnz_test > 0;
@@ -664,15 +785,19 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         // residual into tile order; rows without entries keep y = 0
         if (F.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(F.val, F.tl.tval, F.nnz, F.tl.ntiles, F.tl.sigma, true, c->stream));
         if (S.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(S.val, S.tl.tval, S.nnz, S.tl.ntiles, S.tl.sigma, true, c->stream));
-        HIP_TRY(c, ccd::launch_fill(vF, 0.0f, (uint64_t)dF, c->stream));
-        HIP_TRY(c, ccd::launch_fill(vS, 0.0f, (uint64_t)dS, c->stream));
+        if (bf) {
+            HIP_TRY(c, ccd::launch_fill_bytes16(reinterpret_cast<uint16_t*>(vF), 0, (uint64_t)dF, c->stream));
+            HIP_TRY(c, ccd::launch_fill_bytes16(reinterpret_cast<uint16_t*>(vS), 0, (uint64_t)dS, c->stream));
+        } else {
+            HIP_TRY(c, ccd::launch_fill(reinterpret_cast<float*>(vF), 0.0f, (uint64_t)dF, c->stream));
+            HIP_TRY(c, ccd::launch_fill(reinterpret_cast<float*>(vS), 0.0f, (uint64_t)dS, c->stream));
+        }
     }
     hipEvent_t loop_a = nullptr, loop_b = nullptr;
     if (c->prof) {
         loop_a = take_event(c);
         loop_b = take_event(c);
     }
-    if (p->zero_first) HIP_TRY(c, hipMemsetAsync(XF, 0, sizeof(float) * (size_t)k * dF, c->stream));
     if (loop_a) HIP_TRY(c, hipEventRecord(loop_a, c->stream));
 
     bool pending = false;  // rank `prev`'s final u v^T still to be subtracted
@@ -681,10 +806,10 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     size_t rec_mark = 0;
     for (int oiter = 1; oiter <= T; ++oiter) {
         for (uint32_t t = 0; t < k; ++t) {
-            const float* St = XS + (size_t)t * dS;
-            const float* Ft = XF + (size_t)t * dF;
-            const float* Sp = XS + (size_t)prev * dS;
-            const float* Fp = XF + (size_t)prev * dF;
+            const char* St = XS + (size_t)t * dS * es;
+            const char* Ft = XF + (size_t)t * dF * es;
+            const char* Sp = XS + (size_t)prev * dS * es;
+            const char* Fp = XF + (size_t)prev * dF * es;
             const bool add = oiter > 1 || c->continued;
             const bool fused = pending || add;
             if (Tin >= 1) {
@@ -699,8 +824,8 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
                     if (rc) return rc;
                 }
                 Timed tm(c, CCD_K_COPY);
-                HIP_TRY(c, hipMemcpyAsync(XS + (size_t)t * dS, vS, sizeof(float) * dS, hipMemcpyDeviceToDevice, c->stream));
-                HIP_TRY(c, hipMemcpyAsync(XF + (size_t)t * dF, vF, sizeof(float) * dF, hipMemcpyDeviceToDevice, c->stream));
+                HIP_TRY(c, hipMemcpyAsync(XS + (size_t)t * dS * es, vS, es * dS, hipMemcpyDeviceToDevice, c->stream));
+                HIP_TRY(c, hipMemcpyAsync(XF + (size_t)t * dF * es, vF, es * dF, hipMemcpyDeviceToDevice, c->stream));
             } else if (fused) {
                 rc = pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, pending, add, false, lambda, CCD_K_RESIDUAL);
                 if (rc) return rc;
@@ -712,7 +837,7 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         }
         if (do_rmse) {
             Timed tm(c, CCD_K_RMSE);
-            HIP_TRY(c, ccd::launch_rmse(c->trow, c->tcol, c->tval, c->nnz_test, c->W, c->H, c->m, c->n, k,
+            HIP_TRY(c, ccd::launch_rmse(c->trow, c->tcol, c->tval, c->nnz_test, Wr, Hr, bf, c->m, c->n, k,
                                         c->rmse_bsum, c->rmse_dev + (oiter - 1), c->stream));
         }
         if (cb) {
@@ -733,14 +858,18 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         }
     }
     if (pending) {
-        const float* Sp = XS + (size_t)prev * dS;
-        const float* Fp = XF + (size_t)prev * dF;
+        const char* Sp = XS + (size_t)prev * dS * es;
+        const char* Fp = XF + (size_t)prev * dF * es;
         rc = pass(c, F, nullptr, nullptr, Sp, Fp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
         if (rc) return rc;
         rc = pass(c, S, nullptr, nullptr, Fp, Sp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
         if (rc) return rc;
     }
     if (loop_b) HIP_TRY(c, hipEventRecord(loop_b, c->stream));
+    if (bf) {  // back to the FP32 master copies (exact: every bf16 is a float)
+        HIP_TRY(c, ccd::launch_from_bf16(c->W16, c->W, (uint64_t)k * c->m, c->stream));
+        HIP_TRY(c, ccd::launch_from_bf16(c->H16, c->H, (uint64_t)k * c->n, c->stream));
+    }
     if (tiled) {
         if (F.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(F.tl.tval, F.val, F.nnz, F.tl.ntiles, F.tl.sigma, false, c->stream));
         if (S.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(S.tl.tval, S.val, S.nnz, S.tl.ntiles, S.tl.sigma, false, c->stream));
@@ -768,8 +897,18 @@ int32_t ccd_get_factors(ccd_ctx* c, float* W, float* H) {
     if (!c->factors) return set_err(c, CCD_ERR_STATE, "no factors");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (W) HIP_TRY(c, hipMemcpy(W, c->W, sizeof(float) * (size_t)c->k * c->m, hipMemcpyDeviceToHost));
-    if (H) HIP_TRY(c, hipMemcpy(H, c->H, sizeof(float) * (size_t)c->k * c->n, hipMemcpyDeviceToHost));
+    const uint32_t k = c->k;
+    std::vector<float> t2((size_t)k * std::max(c->m, c->n));
+    if (W) {
+        HIP_TRY(c, hipMemcpy(t2.data(), c->W, sizeof(float) * (size_t)k * c->m, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < k; ++t)
+            for (int64_t i = 0; i < c->m; ++i) W[(size_t)t * c->m + c->csr.perm[i]] = t2[(size_t)t * c->m + i];
+    }
+    if (H) {
+        HIP_TRY(c, hipMemcpy(t2.data(), c->H, sizeof(float) * (size_t)k * c->n, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < k; ++t)
+            for (int64_t j = 0; j < c->n; ++j) H[(size_t)t * c->n + c->csc.perm[j]] = t2[(size_t)t * c->n + j];
+    }
     return CCD_OK;
 }
 
@@ -778,8 +917,18 @@ int32_t ccd_get_residual(ccd_ctx* c, float* csr_val, float* csc_val) {
     if (!c->uploaded) return set_err(c, CCD_ERR_STATE, "nothing uploaded");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (csr_val) HIP_TRY(c, hipMemcpy(csr_val, c->csr.val, 4 * c->nnz, hipMemcpyDeviceToHost));
-    if (csc_val) HIP_TRY(c, hipMemcpy(csc_val, c->csc.val, 4 * c->nnz, hipMemcpyDeviceToHost));
+    std::vector<float> tmp(c->nnz);
+    for (int q = 0; q < 2; ++q) {
+        float* out = q == 0 ? csr_val : csc_val;
+        const DevLayout& L = q == 0 ? c->csr : c->csc;
+        if (!out) continue;
+        HIP_TRY(c, hipMemcpy(tmp.data(), L.val, 4 * c->nnz, hipMemcpyDeviceToHost));
+        // internal row i holds the caller's row perm[i], in the caller's element order
+        for (int64_t i = 0; i < L.rows; ++i) {
+            const uint32_t r = L.perm[i];
+            std::memcpy(out + L.ext_ptr[r], tmp.data() + L.hptr[i], 4 * (size_t)(L.hptr[i + 1] - L.hptr[i]));
+        }
+    }
     return CCD_OK;
 }
 
@@ -792,8 +941,8 @@ int32_t ccd_rmse(ccd_ctx* c, double* out) {
         HIP_TRY(c, dalloc(&c->rmse_dev, 1));
         c->rmse_cap = 1;
     }
-    HIP_TRY(c, ccd::launch_rmse(c->trow, c->tcol, c->tval, c->nnz_test, c->W, c->H, c->m, c->n, c->k, c->rmse_bsum,
-                                c->rmse_dev, c->stream));
+    HIP_TRY(c, ccd::launch_rmse(c->trow, c->tcol, c->tval, c->nnz_test, c->W, c->H, false, c->m, c->n, c->k,
+                                c->rmse_bsum, c->rmse_dev, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(out, c->rmse_dev, sizeof(double), hipMemcpyDeviceToHost));
     return CCD_OK;

@@ -5,15 +5,57 @@
 
 namespace ccd {
 
+// Gather of a factor value: from the LDS-staged hot prefix when c < hot.
+template <class TF>
+__device__ __forceinline__ float gather(const TF* lds, uint32_t hot, const void* g, uint32_t c) {
+    if (c < hot) {
+        if constexpr (sizeof(TF) == 2) return __uint_as_float((uint32_t)lds[c] << 16);
+        else return lds[c];
+    }
+    if constexpr (sizeof(TF) == 2) return __uint_as_float((uint32_t) static_cast<const uint16_t*>(g)[c] << 16);
+    else return static_cast<const float*>(g)[c];
+}
+
 // Native clang vector types: the non-temporal load/store builtins take these
 // (HIP's float4/uint4 are structs).
 typedef float vf4 __attribute__((ext_vector_type(4)));
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
 typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
 
+// Storage of the factor vectors W, H, u, v: float, or bfloat16 kept as its
+// upper 16 bits (bf16 below).  Arithmetic is FP32 either way; per-row sums
+// are accumulated in the accumulator type (float, or double for the
+// FP64-sums check mode).  Precision codes follow enum ccd_precision.
+typedef uint16_t bf16;
+constexpr int PREC_F32 = 0, PREC_BF16 = 1, PREC_F64SUM = 2;
+
+template <class TF>
+__device__ __forceinline__ float ldf(const void* p, size_t i) {
+    if constexpr (sizeof(TF) == 2) return __uint_as_float((uint32_t) static_cast<const bf16*>(p)[i] << 16);
+    else return static_cast<const float*>(p)[i];
+}
+
+// Round to the nearest bfloat16, ties to even; NaN stays a quiet NaN.
+__device__ __forceinline__ bf16 to_bf16(float x) {
+    uint32_t b = __float_as_uint(x);
+    if ((b & 0x7f800000u) == 0x7f800000u) {
+        if (b & 0x007fffffu) b |= 0x00400000u;
+        return (bf16)(b >> 16);
+    }
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (bf16)(b >> 16);
+}
+
+template <class TF>
+__device__ __forceinline__ void stf(void* p, size_t i, float v) {
+    if constexpr (sizeof(TF) == 2) static_cast<bf16*>(p)[i] = to_bf16(v);
+    else static_cast<float*>(p)[i] = v;
+}
+
 // Wave work-unit kinds (uint4 {x, y, z, w}).
-//   GROUP: x = UNIT_GROUP | log2(G) << 8; rows rowlist[y .. z) swept by groups
-//          of G lanes, 64/G rows per pass.
+//   GROUP: x = UNIT_GROUP | log2(G) << 8; rows y .. z-1 swept by groups of G
+//          lanes, 64/G rows per pass (rows are numbered by falling degree, so
+//          a unit is a contiguous row range).
 //   SPLIT: x = UNIT_SPLIT | slot << 8; elements [z, w) of row y; the wave's
 //          partial (g, h) goes to partials[slot].
 constexpr uint32_t UNIT_GROUP = 0;
@@ -22,42 +64,57 @@ constexpr uint32_t UNIT_SPLIT = 2;
 // One sweep (or residual-only pass) over a compressed layout.
 //   residual:  val[e] = (val[e] - xs[idx[e]] * ys[row]) + xa[idx[e]] * ya[row]
 //   sweep:     y[row] = sum x[c] r / (lambda |row| + sum x[c]^2), 0 for empty rows
+// Factor pointers (x, y, xs, ys, xa, ya) point to float or bf16 storage per
+// the precision code; partials hold (g, h) pairs of the accumulator type.
+// The sweep and tiled kernels are persistent: a block first copies the hot
+// prefix x[0 .. hot) of the gathered vector into LDS (dynamic shared memory
+// of hot * sizeof(factor)), then its waves take work units (tiles) in a
+// grid-stride loop.  Indices below `hot` are gathered from LDS, the rest from
+// global memory; xa_is_x lets the add-back gather share the staged copy.
+constexpr int SWEEP_BLOCK = 1024;
+
 struct SweepArgs {
     const uint32_t* ptr;
     const void* idx;
     float* val;
-    const float* x;
-    float* y;
-    const float* xs;
-    const float* ys;
-    const float* xa;
-    const float* ya;
+    const void* x;
+    void* y;
+    const void* xs;
+    const void* ys;
+    const void* xa;
+    const void* ya;
     const uint4* units;
-    const uint32_t* rowlist;
-    float2* partials;
+    void* partials;
     uint32_t nunits;
+    uint32_t hot;
+    uint32_t xa_is_x;
     float lambda;
 };
 
 struct FinalizeArgs {
     const uint32_t* split_rows;
     const uint32_t* split_beg;  // nsplit + 1
-    const float2* partials;
+    const void* partials;
     const uint32_t* ptr;
-    float* y;
+    void* y;
     float lambda;
     uint32_t nsplit;
 };
 
-hipError_t launch_sweep(const SweepArgs& a, bool idx16, bool sub, bool add, bool sweep, hipStream_t s);
-hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
+// blocks: persistent grid size (a multiple of the CU count)
+hipError_t launch_sweep(const SweepArgs& a, int prec, bool idx16, bool sub, bool add, bool sweep, uint32_t blocks,
+                        hipStream_t s);
+hipError_t launch_finalize(const FinalizeArgs& a, int prec, hipStream_t s);
 
 // Test RMSE: per-block FP64 partial sums of squared errors, then one block
 // that reduces them in a fixed order and writes sqrt(sum / nnz) to *out.
 constexpr int RMSE_BLOCKS = 1024;
 hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* tval, uint64_t nnz,
-                       const float* W, const float* H, int64_t m, int64_t n, uint32_t k, double* block_sums,
-                       double* out, hipStream_t s);
+                       const void* W, const void* H, bool bf16_factors, int64_t m, int64_t n, uint32_t k,
+                       double* block_sums, double* out, hipStream_t s);
+// float <-> bfloat16 factor conversion (rounding to nearest even)
+hipError_t launch_to_bf16(const float* src, bf16* dst, uint64_t count, hipStream_t s);
+hipError_t launch_from_bf16(const bf16* src, float* dst, uint64_t count, hipStream_t s);
 
 hipError_t launch_copy(const float4* src, float4* dst, uint64_t count, hipStream_t s);
 
@@ -83,43 +140,48 @@ struct TiledArgs {
     const uint32_t* ptr;
     const void* tidx;
     float* tval;
-    const float* x;
-    float* y;
-    const float* xs;
-    const float* ys;
-    const float* xa;
-    const float* ya;
+    const void* x;
+    void* y;
+    const void* xs;
+    const void* ys;
+    const void* xa;
+    const void* ya;
     const uint32_t* lane_flags;
     const uint16_t* lane_seg0;
     const uint32_t* tile_row0;
     const uint32_t* tile_segoff;
     const uint32_t* segrows;
     const uint8_t* tile_open;
-    float2* head;
-    float2* tail;
+    void* head;  // (g, h) pairs of the accumulator type
+    void* tail;
     uint64_t nnz;
     uint32_t ntiles;
+    uint32_t hot;
+    uint32_t xa_is_x;
     float lambda;
 };
 
 struct SpanArgs {
     const uint32_t* span_rows;
     const uint32_t* ptr;
-    const float2* head;
-    const float2* tail;
-    float* y;
+    const void* head;
+    const void* tail;
+    void* y;
     uint32_t nspan;
     uint32_t tile_elems;
     float lambda;
 };
 
-hipError_t launch_tiled(const TiledArgs& a, int sigma, bool idx16, bool sub, bool add, bool sweep, hipStream_t s);
-hipError_t launch_span(const SpanArgs& a, hipStream_t s);
+// sigma 8, 16 or 32 for FP32; 16 for the other precisions
+hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
+                        uint32_t blocks, hipStream_t s);
+hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s);
 // original order <-> tiled order (to_tiled = true: src original, dst tiled)
 hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_t ntiles, int sigma, bool to_tiled,
                              hipStream_t s);
 hipError_t launch_retile_idx(const void* src, void* dst, bool idx16, uint64_t nnz, uint32_t ntiles, int sigma,
                              hipStream_t s);
 hipError_t launch_fill(float* dst, float value, uint64_t count, hipStream_t s);
+hipError_t launch_fill_bytes16(uint16_t* dst, uint16_t value, uint64_t count, hipStream_t s);
 
 }  // namespace ccd

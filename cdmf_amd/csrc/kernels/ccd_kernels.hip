@@ -11,12 +11,21 @@
 //
 // Work is cut into wave-sized units planned on the host by row degree:
 //   * groups of G = 1..64 lanes per row (lane-per-row for the shortest rows,
-//     wave-per-row for long ones), 64/G rows per pass;
+//     wave-per-row for long ones), 64/G rows per pass; rows are numbered by
+//     falling degree, so every unit is a contiguous row range;
 //   * rows longer than a threshold are split into pieces, each piece one
 //     wave; the per-piece (g, h) partials are reduced in piece order by
 //     finalize_kernel, so results are run-to-run deterministic.
-// Loads of a group are unit-stride across its lanes, so a wave-per-row pass
-// reads 256 B per instruction per stream.  The file is built with
+// Whole-wave ranges (split pieces, wave-per-row groups) read 16-byte groups
+// of four elements per lane; every index/value stream is read and written
+// with non-temporal hints so it does not evict the gathered factor vector
+// from L2.  The kernel is persistent: each 1024-thread block stages the hot
+// prefix of the gathered vector (the highest-degree rows or columns) in LDS
+// and its waves then walk the work units in a grid-stride loop.
+//
+// Template parameters: IdxT (u32 or u16 indices), TF (float or bf16 factor
+// storage), TA (float or double per-row sums), SUB/ADD (residual update),
+// SWEEP (compute the row update).  The file is built with
 // -ffp-contract=off: the residual update rounds the product and the sum
 // separately, exactly like the CPU reference.
 #include "ccd_kernels.h"
@@ -28,16 +37,36 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
-constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
 
-template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
-__device__ __forceinline__ void strided_range(const SweepArgs& a, const IdxT* __restrict__ idx, uint32_t p,
+template <class TA>
+struct Pair {
+    TA g, h;
+};
+
+// Per-block view of the arguments: the LDS copy of the hot prefix of x.
+template <typename TF>
+struct Ctx {
+    const SweepArgs& a;
+    const TF* lds;
+};
+
+template <typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void one_elem(const Ctx<TF>& k, uint32_t c, float& r, float ysr, float yar, TA& g,
+                                         TA& h) {
+    if (SUB) r = r - ldf<TF>(k.a.xs, c) * ysr;
+    if (ADD) r = r + (k.a.xa_is_x ? gather<TF>(k.lds, k.a.hot, k.a.x, c) : ldf<TF>(k.a.xa, c)) * yar;
+    if (SWEEP) {
+        const float xv = gather<TF>(k.lds, k.a.hot, k.a.x, c);
+        g += (TA)xv * (TA)r;
+        h += (TA)xv * (TA)xv;
+    }
+}
+
+template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void strided_range(const Ctx<TF>& k, const IdxT* __restrict__ idx, uint32_t p,
                                               const uint32_t end, const uint32_t step, const float ysr,
-                                              const float yar, float& g, float& h) {
-    float* __restrict__ val = a.val;
-    const float* __restrict__ x = a.x;
-    const float* __restrict__ xs = a.xs;
-    const float* __restrict__ xa = a.xa;
+                                              const float yar, TA& g, TA& h) {
+    float* __restrict__ val = k.a.val;
     // four independent elements per lane in flight
     for (; p + 3 * step < end; p += 4 * step) {
         uint32_t c[4];
@@ -46,47 +75,27 @@ __device__ __forceinline__ void strided_range(const SweepArgs& a, const IdxT* __
         for (int q = 0; q < 4; ++q) c[q] = __builtin_nontemporal_load(idx + p + q * step);
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[q] = __builtin_nontemporal_load(val + p + q * step);
-        if (SUB) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) r[q] = r[q] - xs[c[q]] * ysr;
-        }
-        if (ADD) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r[q] = r[q] + xa[c[q]] * yar;
-        }
+        for (int q = 0; q < 4; ++q) one_elem<TF, TA, SUB, ADD, SWEEP>(k, c[q], r[q], ysr, yar, g, h);
         if (SUB || ADD) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) __builtin_nontemporal_store(r[q], val + p + q * step);
-        }
-        if (SWEEP) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float xv = x[c[q]];
-                g += xv * r[q];
-                h += xv * xv;
-            }
         }
     }
     for (; p < end; p += step) {
         const uint32_t c = __builtin_nontemporal_load(idx + p);
         float r = __builtin_nontemporal_load(val + p);
-        if (SUB) r = r - xs[c] * ysr;
-        if (ADD) r = r + xa[c] * yar;
+        one_elem<TF, TA, SUB, ADD, SWEEP>(k, c, r, ysr, yar, g, h);
         if (SUB || ADD) __builtin_nontemporal_store(r, val + p);
-        if (SWEEP) {
-            const float xv = x[c];
-            g += xv * r;
-            h += xv * xv;
-        }
     }
 }
 
-// Four consecutive elements starting at 4*q (16-byte aligned): one float4
-// value load, one uint4 (or uint2 for 16-bit) index load.
-template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
-__device__ __forceinline__ void group4(const SweepArgs& a, const IdxT* __restrict__ idx, uint32_t q, float ysr,
-                                       float yar, float& g, float& h) {
-    vf4* __restrict__ v4 = reinterpret_cast<vf4*>(a.val) + q;
+// Four consecutive elements starting at 4*q (16-byte aligned): one 16-byte
+// value load, one 16-byte (or 8-byte for 16-bit) index load.
+template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void group4(const Ctx<TF>& k, const IdxT* __restrict__ idx, uint32_t q, float ysr,
+                                       float yar, TA& g, TA& h) {
+    vf4* __restrict__ v4 = reinterpret_cast<vf4*>(k.a.val) + q;
     const vf4 rv = __builtin_nontemporal_load(v4);
     uint32_t c[4];
     if constexpr (sizeof(IdxT) == 2) {
@@ -103,128 +112,127 @@ __device__ __forceinline__ void group4(const SweepArgs& a, const IdxT* __restric
         c[3] = cv.w;
     }
     float r[4] = {rv.x, rv.y, rv.z, rv.w};
-    if (SUB) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) r[j] = r[j] - a.xs[c[j]] * ysr;
-    }
-    if (ADD) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) r[j] = r[j] + a.xa[c[j]] * yar;
-    }
+    for (int j = 0; j < 4; ++j) one_elem<TF, TA, SUB, ADD, SWEEP>(k, c[j], r[j], ysr, yar, g, h);
     if (SUB || ADD) __builtin_nontemporal_store(vf4{r[0], r[1], r[2], r[3]}, v4);
-    if (SWEEP) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float xv = a.x[c[j]];
-            g += xv * r[j];
-            h += xv * xv;
-        }
-    }
 }
 
 // Elements [s, e) by a whole wave: an unaligned head and tail of at most
 // three elements each, then the aligned body as 16-byte groups, two groups
 // (eight elements) per lane in flight.
-template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
-__device__ __forceinline__ void wave_range(const SweepArgs& a, const IdxT* __restrict__ idx, uint32_t s, uint32_t e,
-                                           uint32_t lane, float ysr, float yar, float& g, float& h) {
+template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void wave_range(const Ctx<TF>& k, const IdxT* __restrict__ idx, uint32_t s, uint32_t e,
+                                           uint32_t lane, float ysr, float yar, TA& g, TA& h) {
     uint32_t b0 = (s + 3u) & ~3u;
     if (b0 > e) b0 = e;
     uint32_t b1 = e & ~3u;
     if (b1 < b0) b1 = b0;
-    if (lane < b0 - s) strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, s + lane, s + lane + 1, 1, ysr, yar, g, h);
-    if (lane < e - b1) strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, b1 + lane, b1 + lane + 1, 1, ysr, yar, g, h);
+    if (lane < b0 - s)
+        strided_range<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, s + lane, s + lane + 1, 1, ysr, yar, g, h);
+    if (lane < e - b1)
+        strided_range<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, b1 + lane, b1 + lane + 1, 1, ysr, yar, g, h);
     const uint32_t q1 = b1 >> 2;
     uint32_t q = (b0 >> 2) + lane;
     for (; q + WAVE < q1; q += 2 * WAVE) {
-        group4<IdxT, SUB, ADD, SWEEP>(a, idx, q, ysr, yar, g, h);
-        group4<IdxT, SUB, ADD, SWEEP>(a, idx, q + WAVE, ysr, yar, g, h);
+        group4<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, q, ysr, yar, g, h);
+        group4<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, q + WAVE, ysr, yar, g, h);
     }
-    if (q < q1) group4<IdxT, SUB, ADD, SWEEP>(a, idx, q, ysr, yar, g, h);
+    if (q < q1) group4<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, q, ysr, yar, g, h);
 }
 
-template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
-__global__ __launch_bounds__(BLOCK) void sweep_kernel(SweepArgs a) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t u = blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
-    if (u >= a.nunits) return;
-    const uint4 unit = a.units[u];
+template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void sweep_unit(const Ctx<TF>& k, const uint4 unit, const int lane) {
+    const SweepArgs& a = k.a;
     const IdxT* __restrict__ idx = static_cast<const IdxT*>(a.idx);
-    const uint32_t kind = unit.x & 0xffu;
-
-    if (kind == UNIT_SPLIT) {
+    if ((unit.x & 0xffu) == UNIT_SPLIT) {
         const uint32_t row = unit.y;
-        const float ysr = SUB ? a.ys[row] : 0.0f;
-        const float yar = ADD ? a.ya[row] : 0.0f;
-        float g = 0.0f, h = 0.0f;
-        wave_range<IdxT, SUB, ADD, SWEEP>(a, idx, unit.z, unit.w, (uint32_t)lane, ysr, yar, g, h);
+        const float ysr = SUB ? ldf<TF>(a.ys, row) : 0.0f;
+        const float yar = ADD ? ldf<TF>(a.ya, row) : 0.0f;
+        TA g = 0, h = 0;
+        wave_range<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, unit.z, unit.w, (uint32_t)lane, ysr, yar, g, h);
         if (SWEEP) {
 #pragma unroll
             for (int off = WAVE / 2; off > 0; off >>= 1) {
                 g += __shfl_xor(g, off);
                 h += __shfl_xor(h, off);
             }
-            if (lane == 0) a.partials[unit.x >> 8] = make_float2(g, h);
+            if (lane == 0) static_cast<Pair<TA>*>(a.partials)[unit.x >> 8] = Pair<TA>{g, h};
         }
         return;
     }
-
     const int lg = (int)((unit.x >> 8) & 7u);
     const uint32_t G = 1u << lg;
     const uint32_t sub = (uint32_t)lane >> lg;
     const uint32_t sl = (uint32_t)lane & (G - 1u);
     const uint32_t rows_per_pass = (uint32_t)WAVE >> lg;
     for (uint32_t base = unit.y; base < unit.z; base += rows_per_pass) {
-        const uint32_t ri = base + sub;
-        const bool valid = ri < unit.z;
-        const uint32_t row = valid ? a.rowlist[ri] : 0u;
+        const uint32_t row = base + sub;
+        const bool valid = row < unit.z;
         const uint32_t s = valid ? a.ptr[row] : 0u;
         const uint32_t e = valid ? a.ptr[row + 1] : 0u;
-        const float ysr = (SUB && valid) ? a.ys[row] : 0.0f;
-        const float yar = (ADD && valid) ? a.ya[row] : 0.0f;
-        float g = 0.0f, h = 0.0f;
-        if (lg == 6) wave_range<IdxT, SUB, ADD, SWEEP>(a, idx, s, e, (uint32_t)lane, ysr, yar, g, h);
-        else strided_range<IdxT, SUB, ADD, SWEEP>(a, idx, s + sl, e, G, ysr, yar, g, h);
+        const float ysr = (SUB && valid) ? ldf<TF>(a.ys, row) : 0.0f;
+        const float yar = (ADD && valid) ? ldf<TF>(a.ya, row) : 0.0f;
+        TA g = 0, h = 0;
+        if (lg == 6) wave_range<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, s, e, (uint32_t)lane, ysr, yar, g, h);
+        else strided_range<IdxT, TF, TA, SUB, ADD, SWEEP>(k, idx, s + sl, e, G, ysr, yar, g, h);
         if (SWEEP) {
             for (uint32_t off = G >> 1; off > 0; off >>= 1) {
                 g += __shfl_xor(g, (int)off);
                 h += __shfl_xor(h, (int)off);
             }
-            if (valid && sl == 0) a.y[row] = (e > s) ? g / (a.lambda * (float)(e - s) + h) : 0.0f;
+            if (valid && sl == 0)
+                stf<TF>(a.y, row, (e > s) ? (float)(g / ((TA)a.lambda * (TA)(e - s) + h)) : 0.0f);
         }
     }
 }
 
+template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__global__ __launch_bounds__(SWEEP_BLOCK) void sweep_kernel(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    TF* lds = reinterpret_cast<TF*>(lds_raw);
+    const TF* __restrict__ xg = static_cast<const TF*>(a.x);
+    for (uint32_t i = threadIdx.x; i < a.hot; i += SWEEP_BLOCK) lds[i] = xg[i];
+    __syncthreads();
+    const Ctx<TF> k{a, lds};
+    const int lane = threadIdx.x & (WAVE - 1);
+    constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
+    const uint32_t stride = gridDim.x * NW;
+    for (uint32_t u = blockIdx.x * NW + (threadIdx.x >> 6); u < a.nunits; u += stride)
+        sweep_unit<IdxT, TF, TA, SUB, ADD, SWEEP>(k, a.units[u], lane);
+}
+
+template <typename TF, typename TA>
 __global__ __launch_bounds__(BLOCK) void finalize_kernel(FinalizeArgs a) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= a.nsplit) return;
     const uint32_t row = a.split_rows[i];
-    float g = 0.0f, h = 0.0f;
+    const Pair<TA>* __restrict__ part = static_cast<const Pair<TA>*>(a.partials);
+    TA g = 0, h = 0;
     for (uint32_t q = a.split_beg[i]; q < a.split_beg[i + 1]; ++q) {
-        const float2 pq = a.partials[q];
-        g += pq.x;
-        h += pq.y;
+        g += part[q].g;
+        h += part[q].h;
     }
     const uint32_t d = a.ptr[row + 1] - a.ptr[row];
-    a.y[row] = g / (a.lambda * (float)d + h);
+    stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));
 }
 
 // Per test entry: pred = sum_t W[t][i] * H[t][j] with each product rounded
 // to float and accumulated in double (tools.cpp:556-565); NaN squared errors
 // are skipped but counted (tools.cpp:566-573).
+template <typename TF>
 __global__ __launch_bounds__(BLOCK) void rmse_partial_kernel(const uint32_t* __restrict__ trow,
                                                              const uint32_t* __restrict__ tcol,
                                                              const float* __restrict__ tval, uint64_t nnz,
-                                                             const float* __restrict__ W,
-                                                             const float* __restrict__ H, int64_t m, int64_t n,
-                                                             uint32_t k, double* __restrict__ block_sums) {
+                                                             const void* __restrict__ W, const void* __restrict__ H,
+                                                             int64_t m, int64_t n, uint32_t k,
+                                                             double* __restrict__ block_sums) {
     double acc = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < nnz; e += stride) {
         const uint32_t i = trow[e], j = tcol[e];
         double pred = 0.0;
         for (uint32_t t = 0; t < k; ++t) {
-            const float p = W[(size_t)t * m + i] * H[(size_t)t * n + j];
+            const float p = ldf<TF>(W, (size_t)t * m + i) * ldf<TF>(H, (size_t)t * n + j);
             pred += (double)p;
         }
         const double d = pred - (double)tval[e];
@@ -263,48 +271,92 @@ __global__ __launch_bounds__(BLOCK) void copy_kernel(const float4* __restrict__ 
     for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) dst[i] = src[i];
 }
 
-template <typename IdxT, bool SUB, bool ADD, bool SWEEP>
-hipError_t launch_sweep_t(const SweepArgs& a, hipStream_t s) {
-    const uint32_t blocks = (a.nunits + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((sweep_kernel<IdxT, SUB, ADD, SWEEP>), dim3(blocks), dim3(BLOCK), 0, s, a);
+__global__ __launch_bounds__(BLOCK) void to_bf16_kernel(const float* __restrict__ src, bf16* __restrict__ dst,
+                                                        uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride) dst[i] = to_bf16(src[i]);
+}
+
+__global__ __launch_bounds__(BLOCK) void from_bf16_kernel(const bf16* __restrict__ src, float* __restrict__ dst,
+                                                          uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < count; i += stride)
+        dst[i] = __uint_as_float((uint32_t)src[i] << 16);
+}
+
+uint32_t grid_for(uint64_t count) {
+    uint64_t b = (count + BLOCK - 1) / BLOCK;
+    if (b > 8192) b = 8192;
+    return (uint32_t)(b ? b : 1);
+}
+
+template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+hipError_t launch_sweep_t(const SweepArgs& a, uint32_t blocks, hipStream_t s) {
+    const uint32_t need = (a.nunits + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
+    if (need == 0) return hipSuccess;
+    if (blocks > need) blocks = need;
+    const size_t lds = (size_t)a.hot * sizeof(TF);
+    hipLaunchKernelGGL((sweep_kernel<IdxT, TF, TA, SUB, ADD, SWEEP>), dim3(blocks), dim3(SWEEP_BLOCK), lds, s, a);
     return hipGetLastError();
 }
 
-template <typename IdxT>
-hipError_t launch_sweep_idx(const SweepArgs& a, bool sub, bool add, bool sweep, hipStream_t s) {
+template <typename IdxT, typename TF, typename TA>
+hipError_t launch_sweep_p(const SweepArgs& a, bool sub, bool add, bool sweep, uint32_t blocks, hipStream_t s) {
     const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
     switch (code) {
-        case 1: return launch_sweep_t<IdxT, false, false, true>(a, s);
-        case 2: return launch_sweep_t<IdxT, false, true, false>(a, s);
-        case 3: return launch_sweep_t<IdxT, false, true, true>(a, s);
-        case 4: return launch_sweep_t<IdxT, true, false, false>(a, s);
-        case 5: return launch_sweep_t<IdxT, true, false, true>(a, s);
-        case 6: return launch_sweep_t<IdxT, true, true, false>(a, s);
-        case 7: return launch_sweep_t<IdxT, true, true, true>(a, s);
+        case 1: return launch_sweep_t<IdxT, TF, TA, false, false, true>(a, blocks, s);
+        case 2: return launch_sweep_t<IdxT, TF, TA, false, true, false>(a, blocks, s);
+        case 3: return launch_sweep_t<IdxT, TF, TA, false, true, true>(a, blocks, s);
+        case 4: return launch_sweep_t<IdxT, TF, TA, true, false, false>(a, blocks, s);
+        case 5: return launch_sweep_t<IdxT, TF, TA, true, false, true>(a, blocks, s);
+        case 6: return launch_sweep_t<IdxT, TF, TA, true, true, false>(a, blocks, s);
+        case 7: return launch_sweep_t<IdxT, TF, TA, true, true, true>(a, blocks, s);
         default: return hipSuccess;  // nothing to do
+    }
+}
+
+template <typename IdxT>
+hipError_t launch_sweep_i(const SweepArgs& a, int prec, bool sub, bool add, bool sweep, uint32_t blocks,
+                          hipStream_t s) {
+    switch (prec) {
+        case PREC_F32: return launch_sweep_p<IdxT, float, float>(a, sub, add, sweep, blocks, s);
+        case PREC_BF16: return launch_sweep_p<IdxT, bf16, float>(a, sub, add, sweep, blocks, s);
+        case PREC_F64SUM: return launch_sweep_p<IdxT, float, double>(a, sub, add, sweep, blocks, s);
+        default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-hipError_t launch_sweep(const SweepArgs& a, bool idx16, bool sub, bool add, bool sweep, hipStream_t s) {
-    return idx16 ? launch_sweep_idx<uint16_t>(a, sub, add, sweep, s)
-                 : launch_sweep_idx<uint32_t>(a, sub, add, sweep, s);
+hipError_t launch_sweep(const SweepArgs& a, int prec, bool idx16, bool sub, bool add, bool sweep, uint32_t blocks,
+                        hipStream_t s) {
+    return idx16 ? launch_sweep_i<uint16_t>(a, prec, sub, add, sweep, blocks, s)
+                 : launch_sweep_i<uint32_t>(a, prec, sub, add, sweep, blocks, s);
 }
 
-hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {
+hipError_t launch_finalize(const FinalizeArgs& a, int prec, hipStream_t s) {
     if (a.nsplit == 0) return hipSuccess;
     const uint32_t blocks = (a.nsplit + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(BLOCK), 0, s, a);
+    switch (prec) {
+        case PREC_F32: hipLaunchKernelGGL((finalize_kernel<float, float>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case PREC_BF16: hipLaunchKernelGGL((finalize_kernel<bf16, float>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case PREC_F64SUM:
+            hipLaunchKernelGGL((finalize_kernel<float, double>), dim3(blocks), dim3(BLOCK), 0, s, a);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* tval, uint64_t nnz,
-                       const float* W, const float* H, int64_t m, int64_t n, uint32_t k, double* block_sums,
+hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* tval, uint64_t nnz, const void* W,
+                       const void* H, bool bf16_factors, int64_t m, int64_t n, uint32_t k, double* block_sums,
                        double* out, hipStream_t s) {
-    hipLaunchKernelGGL(rmse_partial_kernel, dim3(RMSE_BLOCKS), dim3(BLOCK), 0, s, trow, tcol, tval, nnz, W, H, m,
-                       n, k, block_sums);
+    if (bf16_factors)
+        hipLaunchKernelGGL(rmse_partial_kernel<bf16>, dim3(RMSE_BLOCKS), dim3(BLOCK), 0, s, trow, tcol, tval, nnz, W,
+                           H, m, n, k, block_sums);
+    else
+        hipLaunchKernelGGL(rmse_partial_kernel<float>, dim3(RMSE_BLOCKS), dim3(BLOCK), 0, s, trow, tcol, tval, nnz, W,
+                           H, m, n, k, block_sums);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rmse_final_kernel, dim3(1), dim3(BLOCK), 0, s, block_sums, RMSE_BLOCKS, nnz, out);
@@ -312,10 +364,20 @@ hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* 
 }
 
 hipError_t launch_copy(const float4* src, float4* dst, uint64_t count, hipStream_t s) {
-    uint64_t blocks = (count + BLOCK - 1) / BLOCK;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(BLOCK), 0, s, src, dst, count);
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(count)), dim3(BLOCK), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float* src, bf16* dst, uint64_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(count)), dim3(BLOCK), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_from_bf16(const bf16* src, float* dst, uint64_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(from_bf16_kernel, dim3(grid_for(count)), dim3(BLOCK), 0, s, src, dst, count);
     return hipGetLastError();
 }
 

@@ -8,6 +8,8 @@
 // layout itself, in the same pass as the sweep, so no re-transposition is
 // ever needed.
 //
+// Persistent kernel: each 1024-thread block stages the hot prefix of the
+// gathered vector in LDS, then its waves take tiles in a grid-stride loop.
 // One wave per tile of 64*SIGMA elements.  Lane l owns elements
 // l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of four,
 // element i of lane l at (i/4)*256 + 4*l + i%4, so one 16-byte load per lane
@@ -35,17 +37,20 @@ __device__ __forceinline__ uint32_t seg_row(uint32_t seg, uint32_t row0, uint32_
     return segoff == 0xffffffffu ? row0 + seg : segrows[segoff + seg];
 }
 
-__device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, float g, float h) {
+template <class TA>
+struct Pair {
+    TA g, h;
+};
+
+template <typename TF, typename TA>
+__device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA g, TA h) {
     const uint32_t d = a.ptr[row + 1] - a.ptr[row];
-    a.y[row] = g / (a.lambda * (float)d + h);
+    stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));
 }
 
-template <typename IdxT, int SIGMA, bool SUB, bool ADD, bool SWEEP>
-__global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const uint32_t tile, const int lane) {
     constexpr uint32_t TS = WAVE * SIGMA;
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t tile = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (tile >= a.ntiles) return;
     const IdxT* __restrict__ tidx = static_cast<const IdxT*>(a.tidx) + (size_t)tile * TS;
     float* __restrict__ tval = a.tval + (size_t)tile * TS;
     const uint32_t flags = a.lane_flags[(size_t)tile * WAVE + lane];
@@ -80,9 +85,9 @@ __global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
 
     uint32_t seg = seg0;
     uint32_t row = seg_row(seg, row0, segoff, a.segrows);
-    float ysr = SUB ? a.ys[row] : 0.0f;
-    float yar = ADD ? a.ya[row] : 0.0f;
-    float g = 0.0f, h = 0.0f, gA = 0.0f, hA = 0.0f;
+    float ysr = SUB ? ldf<TF>(a.ys, row) : 0.0f;
+    float yar = ADD ? ldf<TF>(a.ya, row) : 0.0f;
+    TA g = 0, h = 0, gA = 0, hA = 0;
     bool has_start = (flags & 1u) != 0;
 #pragma unroll
     for (int i = 0; i < SIGMA; ++i) {
@@ -92,26 +97,26 @@ __global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
                     gA = g;
                     hA = h;
                 } else {  // a row that started inside this lane is complete
-                    finish_row(a, row, g, h);
+                    finish_row<TF, TA>(a, row, g, h);
                 }
             }
             has_start = true;
-            g = 0.0f;
-            h = 0.0f;
+            g = 0;
+            h = 0;
             ++seg;
             row = seg_row(seg, row0, segoff, a.segrows);
-            if (SUB) ysr = a.ys[row];
-            if (ADD) yar = a.ya[row];
+            if (SUB) ysr = ldf<TF>(a.ys, row);
+            if (ADD) yar = ldf<TF>(a.ya, row);
         }
         const bool valid = lbase + (uint64_t)i < a.nnz;
         float v = r[i];
-        if (SUB) v = v - a.xs[c[i]] * ysr;
-        if (ADD) v = v + a.xa[c[i]] * yar;
+        if (SUB) v = v - ldf<TF>(a.xs, c[i]) * ysr;
+        if (ADD) v = v + (a.xa_is_x ? gather<TF>(lds, a.hot, a.x, c[i]) : ldf<TF>(a.xa, c[i])) * yar;
         r[i] = v;
         if (SWEEP && valid) {
-            const float xv = a.x[c[i]];
-            g += xv * v;
-            h += xv * xv;
+            const float xv = gather<TF>(lds, a.hot, a.x, c[i]);
+            g += (TA)xv * (TA)v;
+            h += (TA)xv * (TA)xv;
         }
     }
     if (SUB || ADD) {
@@ -125,12 +130,12 @@ __global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
 
     // Segmented inclusive scan over lanes of the lane's contribution to the
     // row open at its end; a lane with a row start begins a new segment.
-    float sg = g, sh = h;
+    TA sg = g, sh = h;
     uint32_t f = has_start ? 1u : 0u;
 #pragma unroll
     for (int d = 1; d < WAVE; d <<= 1) {
-        const float pg = __shfl_up(sg, d);
-        const float ph = __shfl_up(sh, d);
+        const TA pg = __shfl_up(sg, d);
+        const TA ph = __shfl_up(sh, d);
         const uint32_t pf = __shfl_up(f, d);
         if (lane >= d && !f) {
             sg = pg + sg;
@@ -138,11 +143,11 @@ __global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
             f = pf;
         }
     }
-    float cg = __shfl_up(sg, 1);
-    float ch = __shfl_up(sh, 1);
+    TA cg = __shfl_up(sg, 1);
+    TA ch = __shfl_up(sh, 1);
     if (lane == 0) {
-        cg = 0.0f;
-        ch = 0.0f;
+        cg = 0;
+        ch = 0;
     }
     const uint32_t open = a.tile_open[tile];
     if (has_start) {
@@ -150,22 +155,37 @@ __global__ __launch_bounds__(BLOCK) void tiled_kernel(TiledArgs a) {
         if (!(bit0 && lane == 0)) {
             // the row that entered this lane closes at the lane's first row start
             const uint32_t seg_in = bit0 ? seg0 - 1u : seg0;
-            const float tg = bit0 ? cg : cg + gA;
-            const float th = bit0 ? ch : ch + hA;
-            if (seg_in == 0 && (open & 1u)) a.head[tile] = make_float2(tg, th);
-            else finish_row(a, seg_row(seg_in, row0, segoff, a.segrows), tg, th);
+            const TA tg = bit0 ? cg : cg + gA;
+            const TA th = bit0 ? ch : ch + hA;
+            if (seg_in == 0 && (open & 1u)) static_cast<Pair<TA>*>(a.head)[tile] = Pair<TA>{tg, th};
+            else finish_row<TF, TA>(a, seg_row(seg_in, row0, segoff, a.segrows), tg, th);
         }
     }
     if (lane == WAVE - 1) {
         // the row open at the end of the tile
-        if (seg == 0 && (open & 1u)) a.head[tile] = make_float2(sg, sh);
-        else if (open & 2u) a.tail[tile] = make_float2(sg, sh);
-        else finish_row(a, row, sg, sh);
+        if (seg == 0 && (open & 1u)) static_cast<Pair<TA>*>(a.head)[tile] = Pair<TA>{sg, sh};
+        else if (open & 2u) static_cast<Pair<TA>*>(a.tail)[tile] = Pair<TA>{sg, sh};
+        else finish_row<TF, TA>(a, row, sg, sh);
     }
+}
+
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+__global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    TF* lds = reinterpret_cast<TF*>(lds_raw);
+    const TF* __restrict__ xg = static_cast<const TF*>(a.x);
+    for (uint32_t i = threadIdx.x; i < a.hot; i += SWEEP_BLOCK) lds[i] = xg[i];
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1);
+    constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
+    const uint32_t stride = gridDim.x * NW;
+    for (uint32_t tile = blockIdx.x * NW + (threadIdx.x >> 6); tile < a.ntiles; tile += stride)
+        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP>(a, lds, tile, lane);
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], one wave
 // per row, lane-strided partial sums reduced in a fixed tree.
+template <typename TF, typename TA>
 __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t i = blockIdx.x * WPB + (threadIdx.x >> 6);
@@ -173,11 +193,12 @@ __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
     const uint32_t row = a.span_rows[i];
     const uint32_t s = a.ptr[row], e = a.ptr[row + 1];
     const uint32_t t0 = s / a.tile_elems, t1 = (e - 1) / a.tile_elems;
-    float g = 0.0f, h = 0.0f;
+    const Pair<TA>* __restrict__ head = static_cast<const Pair<TA>*>(a.head);
+    TA g = 0, h = 0;
     for (uint32_t t = t0 + 1 + lane; t <= t1; t += WAVE) {
-        const float2 p = a.head[t];
-        g += p.x;
-        h += p.y;
+        const Pair<TA> p = head[t];
+        g += p.g;
+        h += p.h;
     }
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) {
@@ -185,10 +206,10 @@ __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
         h += __shfl_xor(h, off);
     }
     if (lane == 0) {
-        const float2 p = a.tail[t0];
-        g = p.x + g;
-        h = p.y + h;
-        a.y[row] = g / (a.lambda * (float)(e - s) + h);
+        const Pair<TA> p = static_cast<const Pair<TA>*>(a.tail)[t0];
+        g = p.g + g;
+        h = p.h + h;
+        stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)(e - s) + h)));
     }
 }
 
@@ -224,50 +245,74 @@ __global__ __launch_bounds__(BLOCK) void fill_kernel(float* __restrict__ dst, fl
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < count; e += stride) dst[e] = value;
 }
 
+__global__ __launch_bounds__(BLOCK) void fill16_kernel(uint16_t* __restrict__ dst, uint16_t value, uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < count; e += stride) dst[e] = value;
+}
+
 uint32_t grid_for(uint64_t count) {
     uint64_t b = (count + BLOCK - 1) / BLOCK;
     if (b > 16384) b = 16384;
     return (uint32_t)(b ? b : 1);
 }
 
-template <typename IdxT, int SIGMA>
-hipError_t launch_tiled_s(const TiledArgs& a, bool sub, bool add, bool sweep, hipStream_t s) {
-    const uint32_t blocks = (a.ntiles + WPB - 1) / WPB;
-    if (blocks == 0) return hipSuccess;
+template <typename IdxT, int SIGMA, typename TF, typename TA>
+hipError_t launch_tiled_s(const TiledArgs& a, bool sub, bool add, bool sweep, uint32_t blocks, hipStream_t s) {
+    const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
+    if (need == 0) return hipSuccess;
+    if (blocks > need) blocks = need;
+    const size_t lds = (size_t)a.hot * sizeof(TF);
     const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
+#define CCD_TILED(S_, A_, W_) \
+    hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, TF, TA, S_, A_, W_>), dim3(blocks), dim3(SWEEP_BLOCK), lds, s, a)
     switch (code) {
-        case 1: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, false, false, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, false, true, false>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, false, true, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, false, false>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
-        case 5: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, false, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
-        case 6: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, true, false>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
-        case 7: hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, true, true, true>), dim3(blocks), dim3(BLOCK), 0, s, a); break;
+        case 1: CCD_TILED(false, false, true); break;
+        case 2: CCD_TILED(false, true, false); break;
+        case 3: CCD_TILED(false, true, true); break;
+        case 4: CCD_TILED(true, false, false); break;
+        case 5: CCD_TILED(true, false, true); break;
+        case 6: CCD_TILED(true, true, false); break;
+        case 7: CCD_TILED(true, true, true); break;
         default: return hipSuccess;
     }
+#undef CCD_TILED
     return hipGetLastError();
 }
 
 template <typename IdxT>
-hipError_t launch_tiled_i(const TiledArgs& a, int sigma, bool sub, bool add, bool sweep, hipStream_t s) {
-    switch (sigma) {
-        case 8: return launch_tiled_s<IdxT, 8>(a, sub, add, sweep, s);
-        case 16: return launch_tiled_s<IdxT, 16>(a, sub, add, sweep, s);
-        case 32: return launch_tiled_s<IdxT, 32>(a, sub, add, sweep, s);
-        default: return hipErrorInvalidValue;
+hipError_t launch_tiled_i(const TiledArgs& a, int sigma, int prec, bool sub, bool add, bool sweep, uint32_t blocks,
+                          hipStream_t s) {
+    if (prec == PREC_F32) {
+        switch (sigma) {
+            case 8: return launch_tiled_s<IdxT, 8, float, float>(a, sub, add, sweep, blocks, s);
+            case 16: return launch_tiled_s<IdxT, 16, float, float>(a, sub, add, sweep, blocks, s);
+            case 32: return launch_tiled_s<IdxT, 32, float, float>(a, sub, add, sweep, blocks, s);
+            default: return hipErrorInvalidValue;
+        }
     }
+    if (sigma != 16) return hipErrorInvalidValue;
+    if (prec == PREC_BF16) return launch_tiled_s<IdxT, 16, bf16, float>(a, sub, add, sweep, blocks, s);
+    if (prec == PREC_F64SUM) return launch_tiled_s<IdxT, 16, float, double>(a, sub, add, sweep, blocks, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_tiled(const TiledArgs& a, int sigma, bool idx16, bool sub, bool add, bool sweep, hipStream_t s) {
-    return idx16 ? launch_tiled_i<uint16_t>(a, sigma, sub, add, sweep, s)
-                 : launch_tiled_i<uint32_t>(a, sigma, sub, add, sweep, s);
+hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
+                        uint32_t blocks, hipStream_t s) {
+    return idx16 ? launch_tiled_i<uint16_t>(a, sigma, prec, sub, add, sweep, blocks, s)
+                 : launch_tiled_i<uint32_t>(a, sigma, prec, sub, add, sweep, blocks, s);
 }
 
-hipError_t launch_span(const SpanArgs& a, hipStream_t s) {
+hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
     if (a.nspan == 0) return hipSuccess;
-    hipLaunchKernelGGL(tiled_span_kernel, dim3((a.nspan + WPB - 1) / WPB), dim3(BLOCK), 0, s, a);
+    const dim3 grid((a.nspan + WPB - 1) / WPB);
+    switch (prec) {
+        case PREC_F32: hipLaunchKernelGGL((tiled_span_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_BF16: hipLaunchKernelGGL((tiled_span_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_F64SUM: hipLaunchKernelGGL((tiled_span_kernel<float, double>), grid, dim3(BLOCK), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -288,6 +333,12 @@ hipError_t launch_retile_idx(const void* src, void* dst, bool idx16, uint64_t nn
     else
         hipLaunchKernelGGL(retile_idx_kernel<uint32_t>, dim3(grid_for(padded)), dim3(BLOCK), 0, s,
                            static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), nnz, padded, (uint32_t)sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_bytes16(uint16_t* dst, uint16_t value, uint64_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill16_kernel, dim3(grid_for(count)), dim3(BLOCK), 0, s, dst, value, count);
     return hipGetLastError();
 }
 

@@ -311,3 +311,67 @@ def test_tiled_wfirst_and_switching(oracle_lib):
     ref2 = oracle_lib.ccd(ds, W0, H0, c.k, c.lam, 2, c.inner, order=1)
     _compare(W2, H2, [], r_tiled[0], r_tiled[1], *ref2[:2], [], ref2[3], ref2[4], "tiled w-first 2 outer")
     _compare(W, H, res.rmse, rcsr, rcsc, ref[0], ref[1], ref[2][2:], ref[3], ref[4], "then binned 3 outer")
+
+
+# ------------------------------------------------------------ precision modes
+
+def _run_prec(ds, W0, H0, k, lam, outer, inner, precision, variant=1, order=0):
+    solver = _solver()
+    with solver.Solver(0) as s:
+        s.upload(ds)
+        s.set_factors(W0, H0)
+        res = s.run(k, lam, outer, inner, variant=variant, order=order, precision=precision)
+        W, H = s.factors()
+        return W, H, res.rmse, *s.residual()
+
+
+@pytest.mark.parametrize("variant", [1, 3])
+def test_bf16_storage(oracle_lib, variant):
+    """W and H stored in bfloat16, arithmetic in FP32, against the oracle's
+    BF16 twin (same rounding points).  Every stored factor is a bf16 value;
+    a value that lands within rounding noise of a bf16 tie may round the other
+    way, so the bar is test RMSE within 1e-3 and W/H within 1e-2 (Frobenius).
+    The RMSE gap to the FP64 oracle is printed for the precision study."""
+    from cdmf_amd import solver
+    ds = _small()
+    k, lam, outer, inner = 10, 0.05, 4, 3
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    W, H, rm, rcsr, rcsc = _run_prec(ds, W0, H0, k, lam, outer, inner, solver.PREC_BF16, variant=variant)
+    for X in (W, H):
+        assert np.all((X.view(np.uint32) & 0xFFFF) == 0)
+    Wb, Hb, rmb, _, _ = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner, dtype="bf16")
+    _, _, rm64, _, _ = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner, dtype="f64")
+    fw = np.linalg.norm(W - Wb) / np.linalg.norm(Wb)
+    fh = np.linalg.norm(H - Hb) / np.linalg.norm(Hb)
+    _say("[bf16 v%d] |dRMSE| vs bf16 oracle %.3g, vs fp64 oracle %.3g; Frobenius W %.3g H %.3g" % (
+        variant, np.max(np.abs(np.array(rm) - rmb)), np.max(np.abs(np.array(rm) - rm64)), fw, fh))
+    np.testing.assert_allclose(rm, rmb, rtol=0, atol=1e-3)
+    assert fw <= 1e-2 and fh <= 1e-2
+
+
+@pytest.mark.parametrize("variant", [1, 3])
+def test_fp64_sums_against_fp64_oracle(oracle_lib, variant):
+    """FP64 per-row sums (residual still FP32) against the FP64 oracle, to the
+    full elementwise bar."""
+    from cdmf_amd import solver
+    ds = _small()
+    c = (10, 0.05, 5, 5)
+    W0, H0 = oracle_lib.initial_col(c[0], ds.m), oracle_lib.initial_col(c[0], ds.n)
+    got = _run_prec(ds, W0, H0, *c, solver.PREC_FP64_SUMS, variant=variant)
+    W64, H64, rm64, r64, k64 = oracle_lib.ccd(ds, W0, H0, *c, dtype="f64")
+    _compare(*got, W64, H64, rm64, r64, k64, "fp64-sums v%d" % variant)
+
+
+@pytest.mark.skipif(os.environ.get("CDMF_SKIP_LARGE") == "1", reason="large case disabled by CDMF_SKIP_LARGE=1")
+def test_netflix_shape_fp64_sums(oracle_lib):
+    """The full-size case in the FP64-sums check mode: W, H and residual to
+    the strict elementwise bar against the FP64 oracle."""
+    from cdmf_amd import data as D
+    from cdmf_amd import solver
+    ds = D.netflix()
+    k, lam = 2, 0.05
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    got = _run_prec(ds, W0, H0, k, lam, 2, 2, solver.PREC_FP64_SUMS)
+    W64, H64, rm64, r64, k64 = oracle_lib.ccd(ds, W0, H0, k, lam, 2, 2, threads=min(16, os.cpu_count() or 1),
+                                              dtype="f64")
+    _compare(*got, W64, H64, rm64, r64, k64, "netflix fp64-sums")
