@@ -1,7 +1,8 @@
 """ctypes bindings of the two native libraries (include/cdmf_data.h, include/cdmf_hip.h).
 
-The libraries are loaded from cdmf_amd/_lib/ only; nothing here falls back
-to a Python or CPU implementation.  A missing library raises
+The libraries are loaded from cdmf_amd/_lib/ (or the directory named by the
+CDMF_AMD_LIBDIR environment variable, for comparing two builds); nothing here
+falls back to a Python or CPU implementation.  A missing library raises
 NativeLibraryMissing naming the file and the build command.
 """
 from __future__ import annotations
@@ -12,7 +13,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(HERE, "_lib")
+LIBDIR = os.environ.get("CDMF_AMD_LIBDIR") or os.path.join(HERE, "_lib")
 
 
 class NativeLibraryMissing(RuntimeError):

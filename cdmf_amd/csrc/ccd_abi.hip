@@ -71,6 +71,12 @@ struct DevLayout {
     uint32_t nsplit = 0;
     double2* partials = nullptr;  // (g, h) per split piece, double-sized for every precision
     uint32_t npieces = 0;
+    // (previous, current) factor pairs of the gathered side for fused passes:
+    // gdim entries of 8 bytes (FP32) or 4 bytes (BF16)
+    void* pair = nullptr;
+    // LDS split of a fused pass whose sweep gathers a separate x: hot entries
+    // of x and of the pair array (chosen per run for the element size)
+    uint32_t split_x = 0, split_p = 0;
     std::vector<uint32_t> hptr;      // host copy of the internal ptr (planning)
     std::vector<uint32_t> ext_ptr;   // the caller's ptr (residual download)
     std::vector<uint32_t> perm;      // internal row -> caller's row
@@ -178,6 +184,7 @@ void free_layout(DevLayout& L) {
     dfree(L.split_rows);
     dfree(L.split_beg);
     dfree(L.partials);
+    dfree(L.pair);
     L = DevLayout{};
 }
 
@@ -279,20 +286,55 @@ struct RunState {
     const ccd_params* p;
 };
 
-// Entries of the gathered vector staged in LDS: as many of the leading
-// (highest-degree) ones as the per-block LDS budget holds.
-uint32_t hot_count(const ccd_ctx* c, const DevLayout& L, const void* x) {
-    if (!x) return 0;
-    const size_t es = c->prec == CCD_PREC_BF16 ? 2 : 4;
-    const size_t cap = c->lds_budget / es;
-    return (uint32_t)std::min<size_t>(cap, (size_t)L.gdim);
+size_t factor_bytes(const ccd_ctx* c) { return c->prec == CCD_PREC_BF16 ? 2 : 4; }
+
+// Leading (highest-degree) entries of the gathered vectors staged in LDS.
+// A plain sweep stages x only; a fused pass whose x is the pair's current
+// value stages pairs only; a fused pass with a separate x uses the split
+// chosen by choose_split.
+struct Hot {
+    uint32_t x, p;
+};
+
+Hot hot_prefix(const ccd_ctx* c, const DevLayout& L, bool fused, bool sweep, bool x_is_cur) {
+    const size_t es = factor_bytes(c);
+    if (!sweep) return {0, 0};
+    if (!fused) return {(uint32_t)std::min<size_t>(c->lds_budget / es, (size_t)L.gdim), 0};
+    if (x_is_cur) return {0, (uint32_t)std::min<size_t>(c->lds_budget / (2 * es), (size_t)L.gdim)};
+    return {L.split_x, L.split_p};
+}
+
+// Split of the LDS budget between x (es bytes an entry) and the pair array
+// (2 es) that covers the most gathers: entry c of L's gathered vector is
+// gathered deg(c) times, deg(c) being the length of row c of the other
+// layout O, falling with c.  Both prefixes cost one gather per element of
+// L, so the split maximises cum(hx) + cum(hp) with hx + 2 hp <= budget / es.
+void choose_split(const ccd_ctx* c, DevLayout& L, const DevLayout& O) {
+    const size_t cap = c->lds_budget / factor_bytes(c);
+    const size_t g = (size_t)L.gdim;
+    std::vector<uint64_t> cum(g + 1, 0);
+    for (size_t i = 0; i < g; ++i) cum[i + 1] = cum[i] + (O.hptr[i + 1] - O.hptr[i]);
+    uint64_t best = 0;
+    size_t bx = std::min(cap, g), bp = 0;
+    for (size_t hp = 0; 2 * hp <= cap && hp <= g; ++hp) {
+        const size_t hx = std::min(cap - 2 * hp, g);
+        const uint64_t cov = cum[hx] + cum[hp];
+        if (cov > best) {
+            best = cov;
+            bx = hx;
+            bp = hp;
+        }
+    }
+    L.split_x = (uint32_t)bx;
+    L.split_p = (uint32_t)bp;
 }
 
 int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
-            const void* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
-    const void* xg = sweep ? x : nullptr;
-    ccd::SweepArgs a{L.ptr, L.idx, L.val, xg, y, xs, ys, xa, ya, L.units, L.partials, L.nunits,
-                     hot_count(c, L, xg), (uint32_t)(xg && xa == xg), lambda};
+            const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
+    const bool fused = sweep && (sub || add);
+    const Hot hot = hot_prefix(c, L, fused, sweep, x_is_cur);
+    ccd::SweepArgs a{L.ptr, L.idx, L.val, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
+                     L.units, L.partials, L.nunits, hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
         Timed tm(c, kind);
         HIP_TRY(c, ccd::launch_sweep(a, c->prec, L.idx16, sub, add, sweep, c->sweep_blocks, c->stream));
@@ -306,7 +348,7 @@ int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, co
 }
 
 using PassFn = int (*)(ccd_ctx*, DevLayout&, const void*, void*, const void*, const void*, const void*,
-                      const void*, bool, bool, bool, float, int);
+                      const void*, bool, bool, bool, bool, float, int);
 
 int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t nnz, const uint32_t* ptr,
                   const uint32_t* idx, const float* val) {
@@ -438,12 +480,13 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
 }
 
 int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
-                  const void* ya, bool sub, bool add, bool sweep, float lambda, int kind) {
+                  const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
     TiledLayout& T = L.tl;
-    const void* xg = sweep ? x : nullptr;
-    ccd::TiledArgs a{L.ptr, T.tidx, T.tval, xg, y, xs, ys, xa, ya, T.flags, T.seg0, T.row0, T.segoff, T.segrows,
-                     T.open, T.head, T.tail, L.nnz, T.ntiles, hot_count(c, L, xg), (uint32_t)(xg && xa == xg),
-                     lambda};
+    const bool fused = sweep && (sub || add);
+    const Hot hot = hot_prefix(c, L, fused, sweep, x_is_cur);
+    ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
+                     T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.head, T.tail, L.nnz, T.ntiles,
+                     hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
         Timed tm(c, kind);
         HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, c->sweep_blocks, c->stream));
@@ -774,7 +817,7 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     c->kind_bytes[CCD_K_RESIDUAL] = 0.5 * (layout_bytes(F, true, 1, 1, false) + layout_bytes(S, true, 1, 1, false));
     c->kind_bytes[CCD_K_FINALIZE] = 8.0 * (F.npieces + S.npieces) / 2.0;
     c->kind_bytes[CCD_K_RMSE] = 12.0 * (double)c->nnz_test + 4.0 * (double)k * (double)(c->m + c->n);
-    c->kind_bytes[CCD_K_COPY] = 8.0 * (double)(c->m + c->n) / 2.0;
+    c->kind_bytes[CCD_K_COPY] = 5.0 * (p->precision == CCD_PREC_BF16 ? 2.0 : 4.0) * (double)(c->m + c->n);
 
     const bool tiled = p->variant == CCD_VARIANT_TILED;
     const PassFn pass = tiled ? do_tiled_pass : do_pass;
@@ -800,6 +843,26 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     }
     if (loop_a) HIP_TRY(c, hipEventRecord(loop_a, c->stream));
 
+    // Fused passes gather (previous, current) factor pairs: F.pair holds
+    // (XS[prev], XS[t]) for the first-swept layout, S.pair (XF[prev], XF[t])
+    // for the second.  The commit kernel at the end of each rank stores u, v
+    // into W, H and builds the pairs of the next rank; a run that continues a
+    // previous one starts with a fused rank 0 whose pairs are built here.
+    const bool sweeps = Tin >= 1 && k > 0;
+    if (sweeps) {
+        choose_split(c, S, F);
+        for (DevLayout* L : {&F, &S}) {  // sized for FP32 pairs, so every precision fits
+            if (L->pair) continue;
+            double2* d = nullptr;
+            HIP_TRY(c, dalloc(&d, ((size_t)L->gdim + 1) / 2));
+            L->pair = d;
+        }
+        if (c->continued) {
+            ccd::CommitArgs ca{nullptr, nullptr, XS, XS, F.pair, (uint64_t)dS, nullptr, nullptr, XF, XF, S.pair, (uint64_t)dF};
+            HIP_TRY(c, ccd::launch_commit(ca, bf, c->stream));
+        }
+    }
+
     bool pending = false;  // rank `prev`'s final u v^T still to be subtracted
     uint32_t prev = 0;
     int rc;
@@ -812,24 +875,30 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
             const char* Fp = XF + (size_t)prev * dF * es;
             const bool add = oiter > 1 || c->continued;
             const bool fused = pending || add;
-            if (Tin >= 1) {
-                rc = pass(c, F, St, vF, Sp, Fp, St, Ft, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
+            if (sweeps) {
+                rc = pass(c, F, St, vF, Sp, Fp, St, Ft, true, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
                 if (rc) return rc;
-                rc = pass(c, S, vF, vS, Fp, Sp, Ft, St, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
+                rc = pass(c, S, vF, vS, Fp, Sp, Ft, St, false, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
                 if (rc) return rc;
                 for (int it = 2; it <= Tin; ++it) {
-                    rc = pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_F);
+                    rc = pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, false, true, lambda, CCD_K_SWEEP_F);
                     if (rc) return rc;
-                    rc = pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, true, lambda, CCD_K_SWEEP_S);
+                    rc = pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, false, true, lambda, CCD_K_SWEEP_S);
                     if (rc) return rc;
                 }
+                // W, H <- u, v; pairs of the next rank (none after the last)
+                const bool last = oiter == T && t + 1 == k;
+                const uint32_t tn = t + 1 == k ? 0 : t + 1;
+                char* XSt = XS + (size_t)t * dS * es;
+                char* XFt = XF + (size_t)t * dF * es;
+                ccd::CommitArgs ca{vS, XSt, nullptr, XS + (size_t)tn * dS * es, last ? nullptr : F.pair, (uint64_t)dS,
+                                   vF, XFt, nullptr, XF + (size_t)tn * dF * es, last ? nullptr : S.pair, (uint64_t)dF};
                 Timed tm(c, CCD_K_COPY);
-                HIP_TRY(c, hipMemcpyAsync(XS + (size_t)t * dS * es, vS, es * dS, hipMemcpyDeviceToDevice, c->stream));
-                HIP_TRY(c, hipMemcpyAsync(XF + (size_t)t * dF * es, vF, es * dF, hipMemcpyDeviceToDevice, c->stream));
+                HIP_TRY(c, ccd::launch_commit(ca, bf, c->stream));
             } else if (fused) {
-                rc = pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, pending, add, false, lambda, CCD_K_RESIDUAL);
+                rc = pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, false, pending, add, false, lambda, CCD_K_RESIDUAL);
                 if (rc) return rc;
-                rc = pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, pending, add, false, lambda, CCD_K_RESIDUAL);
+                rc = pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, false, pending, add, false, lambda, CCD_K_RESIDUAL);
                 if (rc) return rc;
             }
             pending = true;
@@ -860,9 +929,9 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     if (pending) {
         const char* Sp = XS + (size_t)prev * dS * es;
         const char* Fp = XF + (size_t)prev * dF * es;
-        rc = pass(c, F, nullptr, nullptr, Sp, Fp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
+        rc = pass(c, F, nullptr, nullptr, Sp, Fp, nullptr, nullptr, false, true, false, false, lambda, CCD_K_RESIDUAL);
         if (rc) return rc;
-        rc = pass(c, S, nullptr, nullptr, Fp, Sp, nullptr, nullptr, true, false, false, lambda, CCD_K_RESIDUAL);
+        rc = pass(c, S, nullptr, nullptr, Fp, Sp, nullptr, nullptr, false, true, false, false, lambda, CCD_K_RESIDUAL);
         if (rc) return rc;
     }
     if (loop_b) HIP_TRY(c, hipEventRecord(loop_b, c->stream));

@@ -5,6 +5,24 @@
 
 namespace ccd {
 
+// (previous rank's final value, current rank's value) of one factor entry,
+// interleaved so that a fused residual+sweep pass gathers both in one access.
+template <class TF>
+struct alignas(2 * sizeof(TF)) FPair {
+    TF prev, cur;
+};
+
+template <class TF>
+__device__ __forceinline__ float tof(TF v) {
+    if constexpr (sizeof(TF) == 2) return __uint_as_float((uint32_t)v << 16);
+    else return v;
+}
+
+template <class TF>
+__device__ __forceinline__ FPair<TF> gather_pair(const FPair<TF>* lds, uint32_t hot, const void* g, uint32_t c) {
+    return c < hot ? lds[c] : static_cast<const FPair<TF>*>(g)[c];
+}
+
 // Gather of a factor value: from the LDS-staged hot prefix when c < hot.
 template <class TF>
 __device__ __forceinline__ float gather(const TF* lds, uint32_t hot, const void* g, uint32_t c) {
@@ -67,11 +85,22 @@ constexpr uint32_t UNIT_SPLIT = 2;
 // Factor pointers (x, y, xs, ys, xa, ya) point to float or bf16 storage per
 // the precision code; partials hold (g, h) pairs of the accumulator type.
 // The sweep and tiled kernels are persistent: a block first copies the hot
-// prefix x[0 .. hot) of the gathered vector into LDS (dynamic shared memory
-// of hot * sizeof(factor)), then its waves take work units (tiles) in a
-// grid-stride loop.  Indices below `hot` are gathered from LDS, the rest from
-// global memory; xa_is_x lets the add-back gather share the staged copy.
+// prefixes of its gathered vectors into LDS -- x[0 .. hot_x) and, for fused
+// passes, the pair array xp[0 .. hot_p) -- then its waves take work units
+// (tiles) in a grid-stride loop.  Indices below the hot count are gathered
+// from LDS, the rest from global memory.
+//
+// Fused passes (residual update + sweep) take the subtracted and added
+// factor values of the gathered side from the pair array xp (prev = xs,
+// cur = xa); with x_is_cur the sweep's x is the pair's cur as well (the
+// first-swept layout, whose x is the current rank's old factor).
 constexpr int SWEEP_BLOCK = 1024;
+
+// Byte offset of the staged pair prefix after hot_x factor entries (16-aligned).
+template <class TF>
+__host__ __device__ inline size_t lds_pair_offset(uint32_t hot_x) {
+    return ((size_t)hot_x * sizeof(TF) + 15) & ~(size_t)15;
+}
 
 struct SweepArgs {
     const uint32_t* ptr;
@@ -79,15 +108,17 @@ struct SweepArgs {
     float* val;
     const void* x;
     void* y;
-    const void* xs;
+    const void* xs;  // residual-only passes: gathered vectors as separate arrays
     const void* ys;
     const void* xa;
     const void* ya;
+    const void* xp;  // fused passes: FPair array of the gathered side
     const uint4* units;
     void* partials;
     uint32_t nunits;
-    uint32_t hot;
-    uint32_t xa_is_x;
+    uint32_t hot_x;
+    uint32_t hot_p;
+    uint32_t x_is_cur;
     float lambda;
 };
 
@@ -112,6 +143,27 @@ constexpr int RMSE_BLOCKS = 1024;
 hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* tval, uint64_t nnz,
                        const void* W, const void* H, bool bf16_factors, int64_t m, int64_t n, uint32_t k,
                        double* block_sums, double* out, hipStream_t s);
+// End of a rank: dstS[i] = srcS[i] (i < nS) and dstF[j] = srcF[j] (j < nF) when
+// the sources are given, and the pair arrays of the next rank's fused passes
+//   pairS[i] = (prevS[i], nextS[i]),  pairF[j] = (prevF[j], nextF[j])
+// where prev is the just-finished rank's value (srcS when given, else prevS);
+// a null pair pointer skips that side's pairs.
+struct CommitArgs {
+    const void* srcS;
+    void* dstS;
+    const void* prevS;
+    const void* nextS;
+    void* pairS;
+    uint64_t nS;
+    const void* srcF;
+    void* dstF;
+    const void* prevF;
+    const void* nextF;
+    void* pairF;
+    uint64_t nF;
+};
+hipError_t launch_commit(const CommitArgs& a, bool bf16_factors, hipStream_t s);
+
 // float <-> bfloat16 factor conversion (rounding to nearest even)
 hipError_t launch_to_bf16(const float* src, bf16* dst, uint64_t count, hipStream_t s);
 hipError_t launch_from_bf16(const bf16* src, float* dst, uint64_t count, hipStream_t s);
@@ -146,6 +198,7 @@ struct TiledArgs {
     const void* ys;
     const void* xa;
     const void* ya;
+    const void* xp;
     const uint32_t* lane_flags;
     const uint16_t* lane_seg0;
     const uint32_t* tile_row0;
@@ -156,8 +209,9 @@ struct TiledArgs {
     void* tail;
     uint64_t nnz;
     uint32_t ntiles;
-    uint32_t hot;
-    uint32_t xa_is_x;
+    uint32_t hot_x;
+    uint32_t hot_p;
+    uint32_t x_is_cur;
     float lambda;
 };
 

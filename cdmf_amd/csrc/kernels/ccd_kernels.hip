@@ -43,20 +43,33 @@ struct Pair {
     TA g, h;
 };
 
-// Per-block view of the arguments: the LDS copy of the hot prefix of x.
+// Per-block view of the arguments: the LDS copies of the hot prefixes of x
+// and of the pair array.
 template <typename TF>
 struct Ctx {
     const SweepArgs& a;
     const TF* lds;
+    const FPair<TF>* ldsp;
 };
 
+// One element: residual update (fused passes take both factor values from
+// one pair gather), then the sweep's contribution.
 template <typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
 __device__ __forceinline__ void one_elem(const Ctx<TF>& k, uint32_t c, float& r, float ysr, float yar, TA& g,
                                          TA& h) {
-    if (SUB) r = r - ldf<TF>(k.a.xs, c) * ysr;
-    if (ADD) r = r + (k.a.xa_is_x ? gather<TF>(k.lds, k.a.hot, k.a.x, c) : ldf<TF>(k.a.xa, c)) * yar;
+    constexpr bool PAIR = SWEEP && (SUB || ADD);
+    float xv = 0.0f;
+    if constexpr (PAIR) {
+        const FPair<TF> p = gather_pair<TF>(k.ldsp, k.a.hot_p, k.a.xp, c);
+        if (SUB) r = r - tof<TF>(p.prev) * ysr;
+        if (ADD) r = r + tof<TF>(p.cur) * yar;
+        xv = k.a.x_is_cur ? tof<TF>(p.cur) : gather<TF>(k.lds, k.a.hot_x, k.a.x, c);
+    } else {
+        if (SUB) r = r - ldf<TF>(k.a.xs, c) * ysr;
+        if (ADD) r = r + ldf<TF>(k.a.xa, c) * yar;
+        if (SWEEP) xv = gather<TF>(k.lds, k.a.hot_x, k.a.x, c);
+    }
     if (SWEEP) {
-        const float xv = gather<TF>(k.lds, k.a.hot, k.a.x, c);
         g += (TA)xv * (TA)r;
         h += (TA)xv * (TA)xv;
     }
@@ -190,10 +203,11 @@ template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEE
 __global__ __launch_bounds__(SWEEP_BLOCK) void sweep_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TF* lds = reinterpret_cast<TF*>(lds_raw);
-    const TF* __restrict__ xg = static_cast<const TF*>(a.x);
-    for (uint32_t i = threadIdx.x; i < a.hot; i += SWEEP_BLOCK) lds[i] = xg[i];
+    FPair<TF>* ldsp = reinterpret_cast<FPair<TF>*>(lds_raw + lds_pair_offset<TF>(a.hot_x));
+    for (uint32_t i = threadIdx.x; i < a.hot_x; i += SWEEP_BLOCK) lds[i] = static_cast<const TF*>(a.x)[i];
+    for (uint32_t i = threadIdx.x; i < a.hot_p; i += SWEEP_BLOCK) ldsp[i] = static_cast<const FPair<TF>*>(a.xp)[i];
     __syncthreads();
-    const Ctx<TF> k{a, lds};
+    const Ctx<TF> k{a, lds, ldsp};
     const int lane = threadIdx.x & (WAVE - 1);
     constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
     const uint32_t stride = gridDim.x * NW;
@@ -284,6 +298,29 @@ __global__ __launch_bounds__(BLOCK) void from_bf16_kernel(const bf16* __restrict
         dst[i] = __uint_as_float((uint32_t)src[i] << 16);
 }
 
+template <typename TF>
+__global__ __launch_bounds__(BLOCK) void commit_kernel(CommitArgs a) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < a.nS + a.nF; e += stride) {
+        const bool sside = e < a.nS;
+        const uint64_t i = sside ? e : e - a.nS;
+        const TF* src = static_cast<const TF*>(sside ? a.srcS : a.srcF);
+        TF* dst = static_cast<TF*>(sside ? a.dstS : a.dstF);
+        const TF* prev = static_cast<const TF*>(sside ? a.prevS : a.prevF);
+        const TF* next = static_cast<const TF*>(sside ? a.nextS : a.nextF);
+        FPair<TF>* pair = static_cast<FPair<TF>*>(sside ? a.pairS : a.pairF);
+        const TF nx = pair ? next[i] : TF(0);  // read before dst is written (the same slot when k = 1)
+        TF v;
+        if (src) {
+            v = src[i];
+            dst[i] = v;
+        } else {
+            v = prev[i];
+        }
+        if (pair) pair[i] = FPair<TF>{v, src && next == dst ? v : nx};
+    }
+}
+
 uint32_t grid_for(uint64_t count) {
     uint64_t b = (count + BLOCK - 1) / BLOCK;
     if (b > 8192) b = 8192;
@@ -295,7 +332,7 @@ hipError_t launch_sweep_t(const SweepArgs& a, uint32_t blocks, hipStream_t s) {
     const uint32_t need = (a.nunits + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
     if (need == 0) return hipSuccess;
     if (blocks > need) blocks = need;
-    const size_t lds = (size_t)a.hot * sizeof(TF);
+    const size_t lds = lds_pair_offset<TF>(a.hot_x) + (size_t)a.hot_p * sizeof(FPair<TF>);
     hipLaunchKernelGGL((sweep_kernel<IdxT, TF, TA, SUB, ADD, SWEEP>), dim3(blocks), dim3(SWEEP_BLOCK), lds, s, a);
     return hipGetLastError();
 }
@@ -366,6 +403,14 @@ hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* 
 hipError_t launch_copy(const float4* src, float4* dst, uint64_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(copy_kernel, dim3(grid_for(count)), dim3(BLOCK), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit(const CommitArgs& a, bool bf16_factors, hipStream_t s) {
+    const uint64_t count = a.nS + a.nF;
+    if (count == 0) return hipSuccess;
+    if (bf16_factors) hipLaunchKernelGGL(commit_kernel<bf16>, dim3(grid_for(count)), dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL(commit_kernel<float>, dim3(grid_for(count)), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

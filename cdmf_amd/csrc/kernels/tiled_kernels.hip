@@ -8,8 +8,8 @@
 // layout itself, in the same pass as the sweep, so no re-transposition is
 // ever needed.
 //
-// Persistent kernel: each 1024-thread block stages the hot prefix of the
-// gathered vector in LDS, then its waves take tiles in a grid-stride loop.
+// Persistent kernel: each 1024-thread block stages the hot prefixes of the
+// gathered vectors in LDS, then its waves take tiles in a grid-stride loop.
 // One wave per tile of 64*SIGMA elements.  Lane l owns elements
 // l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of four,
 // element i of lane l at (i/4)*256 + 4*l + i%4, so one 16-byte load per lane
@@ -49,7 +49,9 @@ __device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA 
 }
 
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
-__device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const uint32_t tile, const int lane) {
+__device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
+                                          const int lane) {
+    constexpr bool PAIR = SWEEP && (SUB || ADD);
     constexpr uint32_t TS = WAVE * SIGMA;
     const IdxT* __restrict__ tidx = static_cast<const IdxT*>(a.tidx) + (size_t)tile * TS;
     float* __restrict__ tval = a.tval + (size_t)tile * TS;
@@ -110,11 +112,19 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
         }
         const bool valid = lbase + (uint64_t)i < a.nnz;
         float v = r[i];
-        if (SUB) v = v - ldf<TF>(a.xs, c[i]) * ysr;
-        if (ADD) v = v + (a.xa_is_x ? gather<TF>(lds, a.hot, a.x, c[i]) : ldf<TF>(a.xa, c[i])) * yar;
+        float xv = 0.0f;
+        if constexpr (PAIR) {  // both factor values of the residual update in one gather
+            const FPair<TF> pr = gather_pair<TF>(ldsp, a.hot_p, a.xp, c[i]);
+            if (SUB) v = v - tof<TF>(pr.prev) * ysr;
+            if (ADD) v = v + tof<TF>(pr.cur) * yar;
+            xv = a.x_is_cur ? tof<TF>(pr.cur) : gather<TF>(lds, a.hot_x, a.x, c[i]);
+        } else {
+            if (SUB) v = v - ldf<TF>(a.xs, c[i]) * ysr;
+            if (ADD) v = v + ldf<TF>(a.xa, c[i]) * yar;
+            if (SWEEP) xv = gather<TF>(lds, a.hot_x, a.x, c[i]);
+        }
         r[i] = v;
         if (SWEEP && valid) {
-            const float xv = gather<TF>(lds, a.hot, a.x, c[i]);
             g += (TA)xv * (TA)v;
             h += (TA)xv * (TA)xv;
         }
@@ -173,14 +183,15 @@ template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD
 __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TF* lds = reinterpret_cast<TF*>(lds_raw);
-    const TF* __restrict__ xg = static_cast<const TF*>(a.x);
-    for (uint32_t i = threadIdx.x; i < a.hot; i += SWEEP_BLOCK) lds[i] = xg[i];
+    FPair<TF>* ldsp = reinterpret_cast<FPair<TF>*>(lds_raw + lds_pair_offset<TF>(a.hot_x));
+    for (uint32_t i = threadIdx.x; i < a.hot_x; i += SWEEP_BLOCK) lds[i] = static_cast<const TF*>(a.x)[i];
+    for (uint32_t i = threadIdx.x; i < a.hot_p; i += SWEEP_BLOCK) ldsp[i] = static_cast<const FPair<TF>*>(a.xp)[i];
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
     const uint32_t stride = gridDim.x * NW;
     for (uint32_t tile = blockIdx.x * NW + (threadIdx.x >> 6); tile < a.ntiles; tile += stride)
-        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP>(a, lds, tile, lane);
+        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP>(a, lds, ldsp, tile, lane);
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], one wave
@@ -261,7 +272,7 @@ hipError_t launch_tiled_s(const TiledArgs& a, bool sub, bool add, bool sweep, ui
     const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
     if (need == 0) return hipSuccess;
     if (blocks > need) blocks = need;
-    const size_t lds = (size_t)a.hot * sizeof(TF);
+    const size_t lds = lds_pair_offset<TF>(a.hot_x) + (size_t)a.hot_p * sizeof(FPair<TF>);
     const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
 #define CCD_TILED(S_, A_, W_) \
     hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, TF, TA, S_, A_, W_>), dim3(blocks), dim3(SWEEP_BLOCK), lds, s, a)
