@@ -90,12 +90,13 @@ def main():
     ap.add_argument("--small", action="store_true", help="the 2,000 x 1,000 case instead (quick check)")
     ap.add_argument("--variant", type=int, default=DEFAULT_VARIANT, help="1/2 = degree-binned, 3 = tiled sweeps")
     ap.add_argument("--sigma", type=int, default=16, help="elements per lane of a tile (tiled sweeps)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0,
+                    help="persistent sweep blocks per CU (0: as many as the kernel's registers allow)")
     ap.add_argument("--probe", action="store_true",
                     help="run a 256 MiB stream-copy kernel first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
-              "use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the solver has no multi-GPU path; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -120,7 +121,7 @@ def main():
     H0 = np.zeros((k, ds.n), np.float32)
 
     s = S.Solver(0)
-    s.set_plan(sigma=args.sigma)
+    s.set_plan(sigma=args.sigma, blocks_per_cu=args.blocks_per_cu)
     if args.probe:
         ms = s.copy_probe(256 << 20, reps=5)
         log("copy probe: 256 MiB in %.1f us (%.0f GB/s read+write)" % (ms * 1e3, 2 * (256 << 20) / (ms * 1e-3) / 1e9))

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -74,8 +75,12 @@ struct DevLayout {
     // (previous, current) factor pairs of the gathered side for fused passes:
     // gdim entries of 8 bytes (FP32) or 4 bytes (BF16)
     void* pair = nullptr;
-    // LDS split of a fused pass whose sweep gathers a separate x: hot entries
-    // of x and of the pair array (chosen per run for the element size)
+    // gcum[c]: gathers of entries 0 .. c-1 of the gathered vector per pass
+    // (entry c is gathered once per element of row c of the other layout)
+    std::vector<uint64_t> gcum;
+    // LDS split of a fused pass whose sweep gathers a separate x, for a
+    // budget of split_cap factor entries: hot entries of x and of the pairs
+    uint64_t split_cap = 0;
     uint32_t split_x = 0, split_p = 0;
     std::vector<uint32_t> hptr;      // host copy of the internal ptr (planning)
     std::vector<uint32_t> ext_ptr;   // the caller's ptr (residual download)
@@ -117,13 +122,15 @@ struct ccd_ctx {
     uint16_t* v16 = nullptr;
     uint32_t k16 = 0;
     int prec = CCD_PREC_FP32;  // precision of the run in progress
-    uint32_t sweep_blocks = 0; // persistent grid: 2 blocks of 1024 threads per CU
-    size_t lds_budget = 0;     // LDS bytes per block for the staged hot prefix
+    int ncu = 0;               // compute units
+    int lds_block = 0;         // LDS bytes a block may use
+    int lds_cu = 0;            // LDS bytes of a CU
+    std::map<uint32_t, int> occupancy;  // persistent kernel -> blocks per CU by register use
     double* rmse_bsum = nullptr;
     double* rmse_dev = nullptr;  // one slot per outer iteration
     int32_t rmse_cap = 0;
     std::vector<double> iter_rmse;
-    ccd_plan_params plan{4, 8192, 4096, 4096, 16};
+    ccd_plan_params plan{4, 8192, 4096, 4096, 16, 0};
     // profiling
     bool prof = false;
     std::vector<ProfRec> recs;
@@ -288,56 +295,84 @@ struct RunState {
 
 size_t factor_bytes(const ccd_ctx* c) { return c->prec == CCD_PREC_BF16 ? 2 : 4; }
 
-// Leading (highest-degree) entries of the gathered vectors staged in LDS.
-// A plain sweep stages x only; a fused pass whose x is the pair's current
-// value stages pairs only; a fused pass with a separate x uses the split
-// chosen by choose_split.
-struct Hot {
-    uint32_t x, p;
+// Launch shape of a persistent sweep kernel: as many 1024-thread blocks per
+// CU as its register use allows (at most 2, or the plan's blocks_per_cu),
+// and the LDS those blocks share split evenly between them, 1 KiB spare.
+struct Launch {
+    uint32_t blocks;
+    size_t lds_budget;
 };
 
-Hot hot_prefix(const ccd_ctx* c, const DevLayout& L, bool fused, bool sweep, bool x_is_cur) {
-    const size_t es = factor_bytes(c);
-    if (!sweep) return {0, 0};
-    if (!fused) return {(uint32_t)std::min<size_t>(c->lds_budget / es, (size_t)L.gdim), 0};
-    if (x_is_cur) return {0, (uint32_t)std::min<size_t>(c->lds_budget / (2 * es), (size_t)L.gdim)};
-    return {L.split_x, L.split_p};
+Launch launch_shape(ccd_ctx* c, bool tiled, int sigma, bool idx16, bool sub, bool add, bool sweep, bool xcur) {
+    const uint32_t key = (xcur ? 1u << 17 : 0u) | (tiled ? 1u << 16 : 0u) | (uint32_t)sigma << 8 |
+                         (uint32_t)c->prec << 4 | (idx16 ? 8u : 0u) | (sub ? 4u : 0u) | (add ? 2u : 0u) |
+                         (sweep ? 1u : 0u);
+    auto it = c->occupancy.find(key);
+    int bpc;
+    if (it != c->occupancy.end()) {
+        bpc = it->second;
+    } else {
+        bpc = tiled ? ccd::tiled_blocks_per_cu(sigma, c->prec, idx16, sub, add, sweep, xcur)
+                    : ccd::sweep_blocks_per_cu(c->prec, idx16, sub, add, sweep);
+        c->occupancy[key] = bpc;
+    }
+    bpc = std::max(1, std::min(bpc, 2));
+    if (c->plan.blocks_per_cu) bpc = std::min<int>(bpc, (int)c->plan.blocks_per_cu);
+    const int lds = std::min(c->lds_block, c->lds_cu / bpc) - 1024;
+    return {(uint32_t)(bpc * c->ncu), (size_t)std::max(0, lds) & ~(size_t)255};
 }
 
-// Split of the LDS budget between x (es bytes an entry) and the pair array
-// (2 es) that covers the most gathers: entry c of L's gathered vector is
-// gathered deg(c) times, deg(c) being the length of row c of the other
-// layout O, falling with c.  Both prefixes cost one gather per element of
-// L, so the split maximises cum(hx) + cum(hp) with hx + 2 hp <= budget / es.
-void choose_split(const ccd_ctx* c, DevLayout& L, const DevLayout& O) {
-    const size_t cap = c->lds_budget / factor_bytes(c);
-    const size_t g = (size_t)L.gdim;
-    std::vector<uint64_t> cum(g + 1, 0);
-    for (size_t i = 0; i < g; ++i) cum[i + 1] = cum[i] + (O.hptr[i + 1] - O.hptr[i]);
-    uint64_t best = 0;
-    size_t bx = std::min(cap, g), bp = 0;
-    for (size_t hp = 0; 2 * hp <= cap && hp <= g; ++hp) {
-        const size_t hx = std::min(cap - 2 * hp, g);
-        const uint64_t cov = cum[hx] + cum[hp];
+
+// Split of cap factor entries of LDS between x (one entry each) and the pair
+// array (two) that covers the most gathers: both prefixes cost one gather
+// per element of L, so the split maximises gcum(hx) + gcum(hp) subject to
+// hx + 2 hp <= cap.
+void choose_split(DevLayout& L, uint64_t cap) {
+    if (L.split_cap == cap && cap) return;
+    const uint64_t g = (uint64_t)L.gdim;
+    uint64_t best = 0, bx = std::min(cap, g), bp = 0;
+    for (uint64_t hp = 0; 2 * hp <= cap && hp <= g; ++hp) {
+        const uint64_t hx = std::min(cap - 2 * hp, g);
+        const uint64_t cov = L.gcum[hx] + L.gcum[hp];
         if (cov > best) {
             best = cov;
             bx = hx;
             bp = hp;
         }
     }
+    L.split_cap = cap;
     L.split_x = (uint32_t)bx;
     L.split_p = (uint32_t)bp;
+}
+
+// Leading (highest-degree) entries of the gathered vectors staged in LDS.
+// A plain sweep stages x only; a fused pass whose x is the pair's current
+// value stages pairs only; a fused pass with a separate x splits the budget
+// by choose_split.
+struct Hot {
+    uint32_t x, p;
+};
+
+Hot hot_prefix(const ccd_ctx* c, DevLayout& L, size_t budget, bool fused, bool sweep, bool x_is_cur) {
+    const size_t cap = budget / factor_bytes(c);
+    if (!sweep) return {0, 0};
+    if (!fused) return {(uint32_t)std::min<size_t>(cap, (size_t)L.gdim), 0};
+    if (x_is_cur) return {0, (uint32_t)std::min<size_t>(cap / 2, (size_t)L.gdim)};
+    // the 16-byte alignment of the pair prefix may take up to 15 bytes
+    choose_split(L, cap > 8 ? cap - 8 : 0);
+    return {L.split_x, L.split_p};
 }
 
 int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
             const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
     const bool fused = sweep && (sub || add);
-    const Hot hot = hot_prefix(c, L, fused, sweep, x_is_cur);
+    const Launch sh = launch_shape(c, false, 0, L.idx16, sub, add, sweep, false);
+    const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::SweepArgs a{L.ptr, L.idx, L.val, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      L.units, L.partials, L.nunits, hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
         Timed tm(c, kind);
-        HIP_TRY(c, ccd::launch_sweep(a, c->prec, L.idx16, sub, add, sweep, c->sweep_blocks, c->stream));
+        HIP_TRY(c, ccd::launch_sweep(a, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream));
     }
     if (sweep && L.nsplit) {
         Timed tm(c, CCD_K_FINALIZE);
@@ -357,6 +392,7 @@ int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t
     L.nnz = nnz;
     L.idx16 = gdim <= 65536;
     L.hptr.assign(ptr, ptr + rows + 1);
+    L.split_cap = 0;
     HIP_TRY(c, dalloc(&L.ptr, rows + 1));
     HIP_TRY(c, hipMemcpy(L.ptr, ptr, sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice));
     if (L.idx16) {
@@ -483,13 +519,14 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
                   const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
     TiledLayout& T = L.tl;
     const bool fused = sweep && (sub || add);
-    const Hot hot = hot_prefix(c, L, fused, sweep, x_is_cur);
+    const Launch sh = launch_shape(c, true, T.sigma, L.idx16, sub, add, sweep, fused && x_is_cur);
+    const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
         Timed tm(c, kind);
-        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, c->sweep_blocks, c->stream));
+        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream));
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
@@ -586,9 +623,9 @@ int32_t ccd_create(int32_t device, ccd_ctx** out) {
     if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
         lds_cu <= 0)
         lds_cu = lds_block;
-    c->sweep_blocks = 2u * (uint32_t)ncu;
-    // two persistent blocks per CU share its LDS; keep 1 KiB per block spare
-    c->lds_budget = (size_t)std::max(0, std::min(lds_block, lds_cu / 2) - 1024) & ~(size_t)255;
+    c->ncu = ncu;
+    c->lds_block = lds_block;
+    c->lds_cu = lds_cu;
     if (dalloc(&c->rmse_bsum, ccd::RMSE_BLOCKS) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -631,6 +668,7 @@ int32_t ccd_set_plan(ccd_ctx* c, const ccd_plan_params* pp) {
         return set_err(c, CCD_ERR_INVALID, "plan: elems_per_lane, piece, unit_elems > 0 and split_threshold >= piece");
     if (pp->sigma != 8 && pp->sigma != 16 && pp->sigma != 32)
         return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8, 16 or 32");
+    if (pp->blocks_per_cu > 2) return set_err(c, CCD_ERR_INVALID, "plan: blocks_per_cu must be 0, 1 or 2");
     c->plan = *pp;
     return CCD_OK;
 }
@@ -670,6 +708,11 @@ int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_
         c->csc.ext_ptr.assign(col_ptr, col_ptr + n + 1);
         c->csr.perm = std::move(rperm);
         c->csc.perm = std::move(cperm);
+    }
+    for (DevLayout* L : {&c->csr, &c->csc}) {  // gather counts of each layout's gathered vector
+        const DevLayout& O = L == &c->csr ? c->csc : c->csr;
+        L->gcum.assign((size_t)L->gdim + 1, 0);
+        for (int64_t i = 0; i < L->gdim; ++i) L->gcum[i + 1] = L->gcum[i] + (O.hptr[i + 1] - O.hptr[i]);
     }
     dfree(c->u);
     dfree(c->v);
@@ -850,7 +893,6 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     // previous one starts with a fused rank 0 whose pairs are built here.
     const bool sweeps = Tin >= 1 && k > 0;
     if (sweeps) {
-        choose_split(c, S, F);
         for (DevLayout* L : {&F, &S}) {  // sized for FP32 pairs, so every precision fits
             if (L->pair) continue;
             double2* d = nullptr;

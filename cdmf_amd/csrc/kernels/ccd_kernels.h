@@ -132,10 +132,16 @@ struct FinalizeArgs {
     uint32_t nsplit;
 };
 
+// Dynamic LDS of a persistent sweep: hot_x factor entries, then hot_p pairs.
+size_t staged_lds_bytes(int prec, uint32_t hot_x, uint32_t hot_p);
+
 // blocks: persistent grid size (a multiple of the CU count)
 hipError_t launch_sweep(const SweepArgs& a, int prec, bool idx16, bool sub, bool add, bool sweep, uint32_t blocks,
                         hipStream_t s);
 hipError_t launch_finalize(const FinalizeArgs& a, int prec, hipStream_t s);
+// Blocks of one persistent sweep kernel that fit on a CU by register use
+// (the caller sizes the staged LDS prefix to match).
+int sweep_blocks_per_cu(int prec, bool idx16, bool sub, bool add, bool sweep);
 
 // Test RMSE: per-block FP64 partial sums of squared errors, then one block
 // that reduces them in a fixed order and writes sqrt(sum / nnz) to *out.
@@ -230,6 +236,7 @@ struct SpanArgs {
 hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
                         uint32_t blocks, hipStream_t s);
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s);
+int tiled_blocks_per_cu(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool x_is_cur);
 // original order <-> tiled order (to_tiled = true: src original, dst tiled)
 hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_t ntiles, int sigma, bool to_tiled,
                              hipStream_t s);

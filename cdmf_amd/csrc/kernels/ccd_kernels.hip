@@ -327,48 +327,65 @@ uint32_t grid_for(uint64_t count) {
     return (uint32_t)(b ? b : 1);
 }
 
-template <typename IdxT, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
-hipError_t launch_sweep_t(const SweepArgs& a, uint32_t blocks, hipStream_t s) {
-    const uint32_t need = (a.nunits + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
-    if (need == 0) return hipSuccess;
-    if (blocks > need) blocks = need;
-    const size_t lds = lds_pair_offset<TF>(a.hot_x) + (size_t)a.hot_p * sizeof(FPair<TF>);
-    hipLaunchKernelGGL((sweep_kernel<IdxT, TF, TA, SUB, ADD, SWEEP>), dim3(blocks), dim3(SWEEP_BLOCK), lds, s, a);
-    return hipGetLastError();
-}
+using SweepFn = void (*)(SweepArgs);
 
 template <typename IdxT, typename TF, typename TA>
-hipError_t launch_sweep_p(const SweepArgs& a, bool sub, bool add, bool sweep, uint32_t blocks, hipStream_t s) {
-    const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
+SweepFn sweep_fn_p(int code) {
     switch (code) {
-        case 1: return launch_sweep_t<IdxT, TF, TA, false, false, true>(a, blocks, s);
-        case 2: return launch_sweep_t<IdxT, TF, TA, false, true, false>(a, blocks, s);
-        case 3: return launch_sweep_t<IdxT, TF, TA, false, true, true>(a, blocks, s);
-        case 4: return launch_sweep_t<IdxT, TF, TA, true, false, false>(a, blocks, s);
-        case 5: return launch_sweep_t<IdxT, TF, TA, true, false, true>(a, blocks, s);
-        case 6: return launch_sweep_t<IdxT, TF, TA, true, true, false>(a, blocks, s);
-        case 7: return launch_sweep_t<IdxT, TF, TA, true, true, true>(a, blocks, s);
-        default: return hipSuccess;  // nothing to do
+        case 1: return sweep_kernel<IdxT, TF, TA, false, false, true>;
+        case 2: return sweep_kernel<IdxT, TF, TA, false, true, false>;
+        case 3: return sweep_kernel<IdxT, TF, TA, false, true, true>;
+        case 4: return sweep_kernel<IdxT, TF, TA, true, false, false>;
+        case 5: return sweep_kernel<IdxT, TF, TA, true, false, true>;
+        case 6: return sweep_kernel<IdxT, TF, TA, true, true, false>;
+        case 7: return sweep_kernel<IdxT, TF, TA, true, true, true>;
+        default: return nullptr;
     }
 }
 
 template <typename IdxT>
-hipError_t launch_sweep_i(const SweepArgs& a, int prec, bool sub, bool add, bool sweep, uint32_t blocks,
-                          hipStream_t s) {
+SweepFn sweep_fn_i(int prec, int code) {
     switch (prec) {
-        case PREC_F32: return launch_sweep_p<IdxT, float, float>(a, sub, add, sweep, blocks, s);
-        case PREC_BF16: return launch_sweep_p<IdxT, bf16, float>(a, sub, add, sweep, blocks, s);
-        case PREC_F64SUM: return launch_sweep_p<IdxT, float, double>(a, sub, add, sweep, blocks, s);
-        default: return hipErrorInvalidValue;
+        case PREC_F32: return sweep_fn_p<IdxT, float, float>(code);
+        case PREC_BF16: return sweep_fn_p<IdxT, bf16, float>(code);
+        case PREC_F64SUM: return sweep_fn_p<IdxT, float, double>(code);
+        default: return nullptr;
     }
+}
+
+SweepFn sweep_fn(int prec, bool idx16, bool sub, bool add, bool sweep) {
+    const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
+    return idx16 ? sweep_fn_i<uint16_t>(prec, code) : sweep_fn_i<uint32_t>(prec, code);
 }
 
 }  // namespace
 
+size_t staged_lds_bytes(int prec, uint32_t hot_x, uint32_t hot_p) {
+    const size_t es = prec == PREC_BF16 ? 2 : 4;
+    return (((size_t)hot_x * es + 15) & ~(size_t)15) + (size_t)hot_p * 2 * es;
+}
+
 hipError_t launch_sweep(const SweepArgs& a, int prec, bool idx16, bool sub, bool add, bool sweep, uint32_t blocks,
                         hipStream_t s) {
-    return idx16 ? launch_sweep_i<uint16_t>(a, prec, sub, add, sweep, blocks, s)
-                 : launch_sweep_i<uint32_t>(a, prec, sub, add, sweep, blocks, s);
+    if (!sub && !add && !sweep) return hipSuccess;
+    const SweepFn fn = sweep_fn(prec, idx16, sub, add, sweep);
+    if (!fn) return hipErrorInvalidValue;
+    const uint32_t need = (a.nunits + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
+    if (need == 0) return hipSuccess;
+    if (blocks > need) blocks = need;
+    SweepArgs arg = a;
+    void* args[] = {&arg};
+    return hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
+                           staged_lds_bytes(prec, a.hot_x, a.hot_p), s);
+}
+
+int sweep_blocks_per_cu(int prec, bool idx16, bool sub, bool add, bool sweep) {
+    const SweepFn fn = sweep_fn(prec, idx16, sub, add, sweep);
+    int nb = 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), SWEEP_BLOCK, 0) !=
+                   hipSuccess)
+        return 1;
+    return nb < 1 ? 1 : nb;
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, int prec, hipStream_t s) {

@@ -48,7 +48,8 @@ __device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA 
     stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));
 }
 
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+// XCUR (fused passes only): the sweep's x is the pair's current value.
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
 __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
                                           const int lane) {
     constexpr bool PAIR = SWEEP && (SUB || ADD);
@@ -61,15 +62,11 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
     const uint32_t segoff = a.tile_segoff[tile];
     const uint64_t lbase = (uint64_t)tile * TS + (uint64_t)lane * SIGMA;
 
+    // indices first: the gathers below wait for them, not for the values
     uint32_t c[SIGMA];
     float r[SIGMA];
 #pragma unroll
     for (int q = 0; q < SIGMA / 4; ++q) {
-        const vf4 rv = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(tval + q * 256 + 4 * lane));
-        r[4 * q + 0] = rv.x;
-        r[4 * q + 1] = rv.y;
-        r[4 * q + 2] = rv.z;
-        r[4 * q + 3] = rv.w;
         if constexpr (sizeof(IdxT) == 2) {
             const vu2 cv = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(tidx + q * 256 + 4 * lane));
             c[4 * q + 0] = cv.x & 0xffffu;
@@ -82,6 +79,31 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
             c[4 * q + 1] = cv.y;
             c[4 * q + 2] = cv.z;
             c[4 * q + 3] = cv.w;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SIGMA / 4; ++q) {
+        const vf4 rv = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(tval + q * 256 + 4 * lane));
+        r[4 * q + 0] = rv.x;
+        r[4 * q + 1] = rv.y;
+        r[4 * q + 2] = rv.z;
+        r[4 * q + 3] = rv.w;
+    }
+
+    // Every gather of the lane is issued before the tile's first store, so
+    // all of them are in flight together rather than one per element.
+    float xsv[SIGMA], xav[SIGMA], xv[SIGMA];
+#pragma unroll
+    for (int i = 0; i < SIGMA; ++i) {
+        if constexpr (PAIR) {
+            const FPair<TF> pr = gather_pair<TF>(ldsp, a.hot_p, a.xp, c[i]);
+            xsv[i] = SUB ? tof<TF>(pr.prev) : 0.0f;
+            xav[i] = tof<TF>(pr.cur);
+            xv[i] = XCUR ? xav[i] : gather<TF>(lds, a.hot_x, a.x, c[i]);
+        } else {
+            xsv[i] = SUB ? ldf<TF>(a.xs, c[i]) : 0.0f;
+            xav[i] = ADD ? ldf<TF>(a.xa, c[i]) : 0.0f;
+            xv[i] = SWEEP ? gather<TF>(lds, a.hot_x, a.x, c[i]) : 0.0f;
         }
     }
 
@@ -112,21 +134,12 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
         }
         const bool valid = lbase + (uint64_t)i < a.nnz;
         float v = r[i];
-        float xv = 0.0f;
-        if constexpr (PAIR) {  // both factor values of the residual update in one gather
-            const FPair<TF> pr = gather_pair<TF>(ldsp, a.hot_p, a.xp, c[i]);
-            if (SUB) v = v - tof<TF>(pr.prev) * ysr;
-            if (ADD) v = v + tof<TF>(pr.cur) * yar;
-            xv = a.x_is_cur ? tof<TF>(pr.cur) : gather<TF>(lds, a.hot_x, a.x, c[i]);
-        } else {
-            if (SUB) v = v - ldf<TF>(a.xs, c[i]) * ysr;
-            if (ADD) v = v + ldf<TF>(a.xa, c[i]) * yar;
-            if (SWEEP) xv = gather<TF>(lds, a.hot_x, a.x, c[i]);
-        }
+        if (SUB) v = v - xsv[i] * ysr;
+        if (ADD) v = v + xav[i] * yar;
         r[i] = v;
         if (SWEEP && valid) {
-            g += (TA)xv * (TA)v;
-            h += (TA)xv * (TA)xv;
+            g += (TA)xv[i] * (TA)v;
+            h += (TA)xv[i] * (TA)xv[i];
         }
     }
     if (SUB || ADD) {
@@ -179,7 +192,7 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
     }
 }
 
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP>
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
 __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TF* lds = reinterpret_cast<TF*>(lds_raw);
@@ -191,7 +204,7 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
     const uint32_t stride = gridDim.x * NW;
     for (uint32_t tile = blockIdx.x * NW + (threadIdx.x >> 6); tile < a.ntiles; tile += stride)
-        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP>(a, lds, ldsp, tile, lane);
+        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane);
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], one wave
@@ -267,52 +280,72 @@ uint32_t grid_for(uint64_t count) {
     return (uint32_t)(b ? b : 1);
 }
 
+using TiledFn = void (*)(TiledArgs);
+
+// code: SUB << 2 | ADD << 1 | SWEEP, plus 8 for a fused pass whose x is the
+// pair's current value
 template <typename IdxT, int SIGMA, typename TF, typename TA>
-hipError_t launch_tiled_s(const TiledArgs& a, bool sub, bool add, bool sweep, uint32_t blocks, hipStream_t s) {
-    const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
-    if (need == 0) return hipSuccess;
-    if (blocks > need) blocks = need;
-    const size_t lds = lds_pair_offset<TF>(a.hot_x) + (size_t)a.hot_p * sizeof(FPair<TF>);
-    const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0);
-#define CCD_TILED(S_, A_, W_) \
-    hipLaunchKernelGGL((tiled_kernel<IdxT, SIGMA, TF, TA, S_, A_, W_>), dim3(blocks), dim3(SWEEP_BLOCK), lds, s, a)
+TiledFn tiled_fn_s(int code) {
     switch (code) {
-        case 1: CCD_TILED(false, false, true); break;
-        case 2: CCD_TILED(false, true, false); break;
-        case 3: CCD_TILED(false, true, true); break;
-        case 4: CCD_TILED(true, false, false); break;
-        case 5: CCD_TILED(true, false, true); break;
-        case 6: CCD_TILED(true, true, false); break;
-        case 7: CCD_TILED(true, true, true); break;
-        default: return hipSuccess;
+        case 1: return tiled_kernel<IdxT, SIGMA, TF, TA, false, false, true, false>;
+        case 2: return tiled_kernel<IdxT, SIGMA, TF, TA, false, true, false, false>;
+        case 3: return tiled_kernel<IdxT, SIGMA, TF, TA, false, true, true, false>;
+        case 4: return tiled_kernel<IdxT, SIGMA, TF, TA, true, false, false, false>;
+        case 5: return tiled_kernel<IdxT, SIGMA, TF, TA, true, false, true, false>;
+        case 6: return tiled_kernel<IdxT, SIGMA, TF, TA, true, true, false, false>;
+        case 7: return tiled_kernel<IdxT, SIGMA, TF, TA, true, true, true, false>;
+        case 8 + 3: return tiled_kernel<IdxT, SIGMA, TF, TA, false, true, true, true>;
+        case 8 + 5: return tiled_kernel<IdxT, SIGMA, TF, TA, true, false, true, true>;
+        case 8 + 7: return tiled_kernel<IdxT, SIGMA, TF, TA, true, true, true, true>;
+        default: return nullptr;
     }
-#undef CCD_TILED
-    return hipGetLastError();
 }
 
 template <typename IdxT>
-hipError_t launch_tiled_i(const TiledArgs& a, int sigma, int prec, bool sub, bool add, bool sweep, uint32_t blocks,
-                          hipStream_t s) {
+TiledFn tiled_fn_i(int sigma, int prec, int code) {
     if (prec == PREC_F32) {
         switch (sigma) {
-            case 8: return launch_tiled_s<IdxT, 8, float, float>(a, sub, add, sweep, blocks, s);
-            case 16: return launch_tiled_s<IdxT, 16, float, float>(a, sub, add, sweep, blocks, s);
-            case 32: return launch_tiled_s<IdxT, 32, float, float>(a, sub, add, sweep, blocks, s);
-            default: return hipErrorInvalidValue;
+            case 8: return tiled_fn_s<IdxT, 8, float, float>(code);
+            case 16: return tiled_fn_s<IdxT, 16, float, float>(code);
+            case 32: return tiled_fn_s<IdxT, 32, float, float>(code);
+            default: return nullptr;
         }
     }
-    if (sigma != 16) return hipErrorInvalidValue;
-    if (prec == PREC_BF16) return launch_tiled_s<IdxT, 16, bf16, float>(a, sub, add, sweep, blocks, s);
-    if (prec == PREC_F64SUM) return launch_tiled_s<IdxT, 16, float, double>(a, sub, add, sweep, blocks, s);
-    return hipErrorInvalidValue;
+    if (sigma != 16) return nullptr;
+    if (prec == PREC_BF16) return tiled_fn_s<IdxT, 16, bf16, float>(code);
+    if (prec == PREC_F64SUM) return tiled_fn_s<IdxT, 16, float, double>(code);
+    return nullptr;
+}
+
+TiledFn tiled_fn(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool xcur) {
+    const bool fused = sweep && (sub || add);
+    const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0) | (fused && xcur ? 8 : 0);
+    return idx16 ? tiled_fn_i<uint16_t>(sigma, prec, code) : tiled_fn_i<uint32_t>(sigma, prec, code);
 }
 
 }  // namespace
 
 hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
                         uint32_t blocks, hipStream_t s) {
-    return idx16 ? launch_tiled_i<uint16_t>(a, sigma, prec, sub, add, sweep, blocks, s)
-                 : launch_tiled_i<uint32_t>(a, sigma, prec, sub, add, sweep, blocks, s);
+    if (!sub && !add && !sweep) return hipSuccess;
+    const TiledFn fn = tiled_fn(sigma, prec, idx16, sub, add, sweep, a.x_is_cur != 0);
+    if (!fn) return hipErrorInvalidValue;
+    const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
+    if (need == 0) return hipSuccess;
+    if (blocks > need) blocks = need;
+    TiledArgs arg = a;
+    void* args[] = {&arg};
+    return hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
+                           staged_lds_bytes(prec, a.hot_x, a.hot_p), s);
+}
+
+int tiled_blocks_per_cu(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool xcur) {
+    const TiledFn fn = tiled_fn(sigma, prec, idx16, sub, add, sweep, xcur);
+    int nb = 0;
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), SWEEP_BLOCK, 0) !=
+                   hipSuccess)
+        return 1;
+    return nb < 1 ? 1 : nb;
 }
 
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
