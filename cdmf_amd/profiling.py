@@ -46,18 +46,32 @@ def kernel_stats(d):
 
 
 def kernel_class(name: str):
-    m = re.search(r"(sweep_kernel|tiled_kernel)<(unsigned int|unsigned short), (?:\d+, )?(true|false), (true|false), "
-                  r"(true|false)>", name)
+    """Solver kernel class of a demangled kernel name, or None.
+
+    sweep_kernel<IdxT, TF, TA, SUB, ADD, SWEEP> and
+    tiled_kernel<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>: the first
+    template argument is the index type, the first three booleans are
+    SUB, ADD, SWEEP."""
+    m = re.search(r"(sweep_kernel|tiled_kernel)<([^>]*)>", name)
     if not m:
         if "copy_kernel" in name:
             return "copy_probe"
+        if "commit_kernel" in name:
+            return "copy"
+        if "span_kernel" in name or "finalize_kernel" in name:
+            return "finalize"
+        if "rmse" in name:
+            return "rmse"
         return None
-    kind, idx, sub, add, sweep = m.groups()
-    if sweep != "true":
+    args = [a.strip() for a in m.group(2).split(",")]
+    bools = [a == "true" for a in args if a in ("true", "false")]
+    if len(bools) < 3:
+        return None
+    sub, add, sweep = bools[:3]
+    if not sweep:
         return "residual"
-    layout = "f" if idx == "unsigned int" else "s"
-    fused = sub == "true" or add == "true"
-    return ("fused_" if fused else "sweep_") + layout
+    layout = "f" if args[0] == "unsigned int" else "s"
+    return ("fused_" if (sub or add) else "sweep_") + layout
 
 
 def counters(d, counter):
