@@ -33,7 +33,9 @@ HIP_FLAGS = [
 HIP_SOURCES = [
     "kernels/ccd_kernels.hip",
     "kernels/tiled_kernels.hip",
+    "kernels/batch_kernels.hip",
     "ccd_abi.hip",
+    "batch.hip",
 ]
 HOST_SOURCES = [
     "host/tools.cpp",

@@ -1,0 +1,248 @@
+// Batched CCD++: many independent FP32 factorizations in one persistent
+// launch (the batching extension; the reference solves one matrix per
+// process, src/main.cpp:38-64).
+//
+// Every problem owns a team of blocks.  A team runs its problem's whole
+// outer/rank/inner loop exactly as ccd_run does for one matrix: the same
+// tiled passes (tile_body, shared with tiled_kernels.hip), span finishes,
+// rank commits and final subtraction, in the same order and with the same
+// tile-to-lane assignment, so every problem's W, H and residual are
+// bit-identical to a single-problem tiled run with the same sigma.  Between
+// dependent phases the team meets at a barrier: a per-problem counter in
+// global memory that each block bumps once and waits on (monotonic targets,
+// so it is never reset inside the kernel).  Teams never wait for each other.
+//
+// Termination: the host sizes the grid to what is co-resident on the device
+// (occupancy query), and every barrier wait is bounded by wall_clock64(); a
+// wait that runs out sets *status and the whole team leaves the kernel, so
+// the grid always drains.
+//
+// Block placement: workgroups are dealt to the 8 XCDs round-robin by block
+// index, so virtual block id v = (b % 8) * (B / 8) + b / 8 puts consecutive
+// virtual ids on one XCD; teams are contiguous virtual ranges and a small
+// problem's gathered vectors stay in one XCD's L2.
+#include "tile_body.h"
+
+namespace ccd {
+namespace {
+
+constexpr int WAVE = tiled::WAVE;
+constexpr int WPB = BATCH_BLOCK / WAVE;
+
+struct Team {
+    uint32_t* cnt;
+    uint32_t nb;
+    uint32_t target;
+    int* status;
+    uint64_t wait_ticks;
+};
+
+// Barrier of the blocks of one team.  false: the wait ran out (status set);
+// the caller returns.
+__device__ __forceinline__ bool team_sync(Team& t) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (t.nb == 1) return true;
+    t.target += t.nb;
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        __hip_atomic_fetch_add(t.cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const long long t0 = wall_clock64();
+        while (__hip_atomic_load(t.cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < t.target) {
+            __builtin_amdgcn_s_sleep(2);
+            if ((uint64_t)(wall_clock64() - t0) > t.wait_ticks) {
+                __hip_atomic_store(t.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// The gathered and row-side vectors of one pass.
+struct PassVecs {
+    const float* x;
+    float* y;
+    const float* xs;
+    const float* ys;
+    const float* xa;
+    const float* ya;
+    uint32_t xcur;
+    float lambda;
+};
+
+__device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const PassVecs& v) {
+    TiledArgs a;
+    a.ptr = L.ptr;
+    a.tidx = L.tidx;
+    a.tval = L.tval;
+    a.x = v.x;
+    a.y = v.y;
+    a.xs = v.xs;
+    a.ys = v.ys;
+    a.xa = v.xa;
+    a.ya = v.ya;
+    a.xp = L.pair;
+    a.lane_flags = L.lane_flags;
+    a.lane_seg0 = L.lane_seg0;
+    a.tile_row0 = L.tile_row0;
+    a.tile_segoff = L.tile_segoff;
+    a.segrows = L.segrows;
+    a.tile_open = L.tile_open;
+    a.head = L.head;
+    a.tail = L.tail;
+    a.nnz = L.nnz;
+    a.ntiles = L.ntiles;
+    a.hot_x = 0;  // no LDS staging: a team's gathered vectors stay in its XCD's L2
+    a.hot_p = 0;
+    a.x_is_cur = v.xcur;
+    a.lambda = v.lambda;
+    return a;
+}
+
+// The tiles of one pass taken by wave w of nw.
+template <int SIGMA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
+__device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, uint32_t w, uint32_t nw) {
+    const BatchLayout& L = *Lp;
+    const TiledArgs a = layout_args(L, v);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (L.idx16) {
+        for (uint32_t t = w; t < L.ntiles; t += nw)
+            tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR>(a, nullptr, nullptr, t, lane);
+    } else {
+        for (uint32_t t = w; t < L.ntiles; t += nw)
+            tiled::tile_body<uint32_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR>(a, nullptr, nullptr, t, lane);
+    }
+}
+
+// Modes used by the loop: plain sweep, fused sweeps (subtract, or subtract
+// and add; x = pair's current value or separate), the add-back alone and the
+// final subtraction.  (The first rank of a continued run adds back in a pass
+// of its own followed by plain sweeps: the same operations in the same order
+// as the fused add-only pass of the single-problem path, which is left out
+// here because it makes the compiler spill the whole kernel.)
+template <int SIGMA>
+__device__ __forceinline__ void tiles(const BatchLayout* L, const PassVecs& v, bool sub, bool add, bool sweep,
+                                      uint32_t w, uint32_t nw) {
+    const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0) | (sweep && (sub || add) && v.xcur ? 8 : 0);
+    switch (code) {
+        case 1: tiles_t<SIGMA, false, false, true, false>(L, v, w, nw); break;
+        case 2: tiles_t<SIGMA, false, true, false, false>(L, v, w, nw); break;
+        case 4: tiles_t<SIGMA, true, false, false, false>(L, v, w, nw); break;
+        case 5: tiles_t<SIGMA, true, false, true, false>(L, v, w, nw); break;
+        case 7: tiles_t<SIGMA, true, true, true, false>(L, v, w, nw); break;
+        case 8 + 5: tiles_t<SIGMA, true, false, true, true>(L, v, w, nw); break;
+        case 8 + 7: tiles_t<SIGMA, true, true, true, true>(L, v, w, nw); break;
+        default: break;  // not produced by the loop below
+    }
+}
+
+template <int SIGMA>
+__global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
+    const uint32_t v = (blockIdx.x % 8u) * (ba.nblocks / 8u) + blockIdx.x / 8u;
+    const uint32_t p = ba.block_problem[v];
+    if (p >= ba.nprob) return;
+    const BatchProblem& P = ba.probs[p];
+    const uint32_t rank = v - P.first;
+    Team tm{ba.barriers + p, P.nblocks, 0u, ba.status, ba.wait_ticks};
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t w = rank * WPB + threadIdx.x / WAVE, nw = P.nblocks * WPB;
+    const uint32_t tid = rank * BATCH_BLOCK + threadIdx.x, nth = P.nblocks * BATCH_BLOCK;
+    const uint32_t dF = P.F.rows, dS = P.S.rows, TE = WAVE * SIGMA;
+    const float lambda = P.lambda;
+
+    // one sweep (or fused) pass: tiles, barrier, rows spanning tiles, barrier
+    auto pass = [&](const BatchLayout& L, const PassVecs& pv, bool sub, bool add) -> bool {
+        tiles<SIGMA>(&L, pv, sub, add, true, w, nw);
+        if (!team_sync(tm)) return false;
+        if (L.nspan) {
+            const SpanArgs sa{L.span_rows, L.ptr, L.head, L.tail, pv.y, L.nspan, TE, lambda};
+            for (uint32_t i = w; i < L.nspan; i += nw) tiled::span_row<float, float>(sa, i, lane);
+            if (!team_sync(tm)) return false;
+        }
+        return true;
+    };
+    auto commit = [&](const CommitArgs& ca) -> bool {
+        for (uint64_t e = tid; e < ca.nS + ca.nF; e += nth) commit_elem<float>(ca, e);
+        return team_sync(tm);
+    };
+    auto plain = [&](const float* x, float* y) { return PassVecs{x, y, nullptr, nullptr, nullptr, nullptr, 0u, lambda}; };
+
+    if (ba.trial) {  // autotuner: plain sweeps of rank 0; no state changes but vF, vS
+        for (uint32_t it = 0; it < P.trial_reps; ++it) {
+            if (!pass(P.F, plain(P.XS, P.vF), false, false)) return;
+            if (!pass(P.S, plain(P.vF, P.vS), false, false)) return;
+        }
+        return;
+    }
+
+    bool pending = false;  // rank prev's final u v^T still to be subtracted
+    uint32_t prev = 0;
+    for (uint32_t oiter = 1; oiter <= P.T; ++oiter) {
+        for (uint32_t t = 0; t < P.k; ++t) {
+            const float* St = P.XS + (size_t)t * dS;
+            const float* Ft = P.XF + (size_t)t * dF;
+            const float* Sp = P.XS + (size_t)prev * dS;
+            const float* Fp = P.XF + (size_t)prev * dF;
+            const bool add = oiter > 1 || P.continued;
+            if (add && !pending) {  // rank 0 of a continued run: add back, then plain sweeps
+                tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, nullptr, nullptr, St, Ft, 0u, lambda}, false, true, false,
+                             w, nw);
+                tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, nullptr, nullptr, Ft, St, 0u, lambda}, false, true, false,
+                             w, nw);
+                // each wave sweeps the tiles it just updated: no barrier needed
+                if (!pass(P.F, plain(St, P.vF), false, false)) return;
+                if (!pass(P.S, plain(P.vF, P.vS), false, false)) return;
+            } else {
+                if (!pass(P.F, PassVecs{St, P.vF, Sp, Fp, St, Ft, 1u, lambda}, pending, add)) return;
+                if (!pass(P.S, PassVecs{P.vF, P.vS, Fp, Sp, Ft, St, 0u, lambda}, pending, add)) return;
+            }
+            for (uint32_t it = 2; it <= P.Tin; ++it) {
+                if (!pass(P.F, plain(P.vS, P.vF), false, false)) return;
+                if (!pass(P.S, plain(P.vF, P.vS), false, false)) return;
+            }
+            const bool last = oiter == P.T && t + 1 == P.k;
+            const uint32_t tn = t + 1 == P.k ? 0u : t + 1;
+            const CommitArgs ca{P.vS, P.XS + (size_t)t * dS, nullptr, P.XS + (size_t)tn * dS,
+                                last ? nullptr : P.F.pair, dS,
+                                P.vF, P.XF + (size_t)t * dF, nullptr, P.XF + (size_t)tn * dF,
+                                last ? nullptr : P.S.pair, dF};
+            if (!commit(ca)) return;
+            pending = true;
+            prev = t;
+        }
+    }
+    if (pending) {  // rank prev's final subtraction; the two layouts are independent
+        const float* Sp = P.XS + (size_t)prev * dS;
+        const float* Fp = P.XF + (size_t)prev * dF;
+        tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, Sp, Fp, nullptr, nullptr, 0u, lambda}, true, false, false, w, nw);
+        tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, Fp, Sp, nullptr, nullptr, 0u, lambda}, true, false, false, w, nw);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch(const BatchArgs& a, int sigma, hipStream_t s) {
+    if (a.nblocks == 0 || a.nblocks % 8u) return hipErrorInvalidValue;
+    switch (sigma) {
+        case 8: hipLaunchKernelGGL(batch_kernel<8>, dim3(a.nblocks), dim3(BATCH_BLOCK), 0, s, a); break;
+        case 16: hipLaunchKernelGGL(batch_kernel<16>, dim3(a.nblocks), dim3(BATCH_BLOCK), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int sigma) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (sigma == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<8>, BATCH_BLOCK, 0);
+    if (sigma == 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<16>, BATCH_BLOCK, 0);
+    return e == hipSuccess && nb > 0 ? nb : 0;
+}
+
+}  // namespace ccd

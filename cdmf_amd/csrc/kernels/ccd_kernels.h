@@ -170,6 +170,28 @@ struct CommitArgs {
 };
 hipError_t launch_commit(const CommitArgs& a, bool bf16_factors, hipStream_t s);
 
+// Entry e of a commit: e < nS is entry e of the S side, else entry e - nS
+// of the F side.
+template <typename TF>
+__device__ __forceinline__ void commit_elem(const CommitArgs& a, uint64_t e) {
+    const bool sside = e < a.nS;
+    const uint64_t i = sside ? e : e - a.nS;
+    const TF* src = static_cast<const TF*>(sside ? a.srcS : a.srcF);
+    TF* dst = static_cast<TF*>(sside ? a.dstS : a.dstF);
+    const TF* prev = static_cast<const TF*>(sside ? a.prevS : a.prevF);
+    const TF* next = static_cast<const TF*>(sside ? a.nextS : a.nextF);
+    FPair<TF>* pair = static_cast<FPair<TF>*>(sside ? a.pairS : a.pairF);
+    const TF nx = pair ? next[i] : TF(0);  // read before dst is written (the same slot when k = 1)
+    TF v;
+    if (src) {
+        v = src[i];
+        dst[i] = v;
+    } else {
+        v = prev[i];
+    }
+    if (pair) pair[i] = FPair<TF>{v, src && next == dst ? v : nx};
+}
+
 // float <-> bfloat16 factor conversion (rounding to nearest even)
 hipError_t launch_to_bf16(const float* src, bf16* dst, uint64_t count, hipStream_t s);
 hipError_t launch_from_bf16(const bf16* src, float* dst, uint64_t count, hipStream_t s);
@@ -244,5 +266,65 @@ hipError_t launch_retile_idx(const void* src, void* dst, bool idx16, uint64_t nn
                              hipStream_t s);
 hipError_t launch_fill(float* dst, float value, uint64_t count, hipStream_t s);
 hipError_t launch_fill_bytes16(uint16_t* dst, uint16_t value, uint64_t count, hipStream_t s);
+
+// --------------------------------------------------------------- batched ---
+// Many independent FP32 factorizations in one persistent launch.  Each
+// problem owns a team of consecutive (XCD-local where possible) blocks that
+// runs its whole CCD++ loop -- the same tiled passes, span finishes and
+// commits as the single-problem path, in the same order -- separated by
+// team barriers (a per-problem counter in global memory).  Problems never
+// wait for each other.  The grid never exceeds what is co-resident on the
+// device, and every barrier wait is bounded in time: a wait that runs out
+// sets *status and the team leaves the kernel.
+constexpr int BATCH_BLOCK = 256;
+
+struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
+    const uint32_t* ptr;
+    const void* tidx;
+    float* tval;
+    const uint32_t* lane_flags;
+    const uint16_t* lane_seg0;
+    const uint32_t* tile_row0;
+    const uint32_t* tile_segoff;
+    const uint32_t* segrows;
+    const uint8_t* tile_open;
+    void* head;
+    void* tail;
+    const uint32_t* span_rows;
+    void* pair;  // FPair<float>[gathered length]
+    uint64_t nnz;
+    uint32_t ntiles;
+    uint32_t nspan;
+    uint32_t idx16;
+    uint32_t rows;
+};
+
+struct BatchProblem {
+    BatchLayout F, S;  // first- and second-swept layouts
+    float* XF;         // [k][F.rows] factor produced by the F sweeps
+    float* XS;         // [k][S.rows]
+    float* vF;         // work vectors
+    float* vS;
+    uint32_t k, T, Tin;
+    uint32_t continued;  // residual already holds R - W H^T: add back from the first rank on
+    uint32_t first, nblocks;  // team: virtual block ids [first, first + nblocks)
+    uint32_t trial_reps;      // autotuner trial: plain sweep pairs of rank 0 to run
+    float lambda;
+};
+
+struct BatchArgs {
+    const BatchProblem* probs;
+    const uint32_t* block_problem;  // virtual block id -> problem (0xffffffff: idle)
+    uint32_t nprob;
+    uint32_t nblocks;               // grid size (a multiple of 8)
+    uint32_t* barriers;             // one counter per problem, zero at launch
+    int* status;                    // 0; 1 after a barrier wait ran out
+    uint64_t wait_ticks;            // bound of one barrier wait in wall_clock64() ticks
+    uint32_t trial;                 // autotuner trial: trial_reps plain sweep pairs per problem, nothing else
+};
+
+// sigma 8 or 16
+hipError_t launch_batch(const BatchArgs& a, int sigma, hipStream_t s);
+int batch_blocks_per_cu(int sigma);
 
 }  // namespace ccd

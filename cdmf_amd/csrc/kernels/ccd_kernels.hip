@@ -301,24 +301,7 @@ __global__ __launch_bounds__(BLOCK) void from_bf16_kernel(const bf16* __restrict
 template <typename TF>
 __global__ __launch_bounds__(BLOCK) void commit_kernel(CommitArgs a) {
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < a.nS + a.nF; e += stride) {
-        const bool sside = e < a.nS;
-        const uint64_t i = sside ? e : e - a.nS;
-        const TF* src = static_cast<const TF*>(sside ? a.srcS : a.srcF);
-        TF* dst = static_cast<TF*>(sside ? a.dstS : a.dstF);
-        const TF* prev = static_cast<const TF*>(sside ? a.prevS : a.prevF);
-        const TF* next = static_cast<const TF*>(sside ? a.nextS : a.nextF);
-        FPair<TF>* pair = static_cast<FPair<TF>*>(sside ? a.pairS : a.pairF);
-        const TF nx = pair ? next[i] : TF(0);  // read before dst is written (the same slot when k = 1)
-        TF v;
-        if (src) {
-            v = src[i];
-            dst[i] = v;
-        } else {
-            v = prev[i];
-        }
-        if (pair) pair[i] = FPair<TF>{v, src && next == dst ? v : nx};
-    }
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < a.nS + a.nF; e += stride) commit_elem<TF>(a, e);
 }
 
 uint32_t grid_for(uint64_t count) {

@@ -47,6 +47,55 @@ ML20M = dict(m=138493, n=26744, nnz_train=18000237, nnz_test=2000026, max_row_de
 ML20M_SOLVE = SolveConfig(k=64, lam=0.05, outer=10, inner=5)
 
 
+# The batched case: 64 independent problems, rows log-uniform in [1k, 200k],
+# columns log-uniform in [500, 20k], ratings log-uniform in [10k, 5M]
+# (power-law degrees, integer ratings 1..5, 10 % held out), k from
+# {8, 16, 32, 64}, lambda from {0.01, 0.05, 0.1}, 8 outer x 5 inner.  Two
+# adjustments keep every draw a valid power-law matrix: at most half of the
+# cells are rated, and a problem with fewer than two ratings per row keeps
+# only nnz / 2 rows.
+BATCH = dict(count=64, rows=(1000, 200000), cols=(500, 20000), nnz=(10000, 5000000), ks=(8, 16, 32, 64),
+             lambdas=(0.01, 0.05, 0.1), outer=8, inner=5, seed=4)
+
+
+@dataclass(frozen=True)
+class BatchProblem:
+    m: int
+    n: int
+    nnz: int  # train + test
+    seed: int
+    solve: SolveConfig
+
+
+def batch_specs(count: int = BATCH["count"], seed: int = BATCH["seed"], scale: float = 1.0) -> list:
+    """The shapes and hyper-parameters of the batched case (`scale` shrinks
+    rows, columns and ratings for quick checks)."""
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi):
+        return math.exp(rng.uniform(math.log(lo), math.log(hi)))
+
+    out = []
+    for i in range(count):
+        m = max(8, int(logu(*BATCH["rows"]) * scale))
+        n = max(8, int(logu(*BATCH["cols"]) * scale))
+        nnz = max(64, int(logu(*BATCH["nnz"]) * scale))
+        nnz = min(nnz, (m * n) // 2)
+        if nnz < 2 * m:
+            m = max(8, nnz // 2)
+            nnz = min(nnz, (m * n) // 2)
+        k = int(rng.choice(BATCH["ks"]))
+        lam = float(rng.choice(BATCH["lambdas"]))
+        out.append(BatchProblem(m, n, nnz, seed * 1000 + i, SolveConfig(k, lam, BATCH["outer"], BATCH["inner"])))
+    return out
+
+
+def batch_dataset(bp: BatchProblem, threads: int = 0) -> Dataset:
+    test = bp.nnz // 10
+    return gen_powerlaw(bp.m, bp.n, bp.nnz - test, test, max_row_degree=bp.n, col_exponent=1.0, rating_kind=0,
+                        seed=bp.seed, threads=threads)
+
+
 def gen_uniform(m: int, n: int, nnz_train: int, nnz_test: int, seed: int = 0,
                 rating_kind: int = 0) -> Dataset:
     cd = N.CDataset()

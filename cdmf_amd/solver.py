@@ -43,6 +43,10 @@ class CPlan(C.Structure):
                 ("unit_elems", C.c_uint32), ("sigma", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
 
 
+class CBatchPlan(C.Structure):
+    _fields_ = [("sigma", C.c_uint32), ("blocks", C.c_uint32), ("work_exponent", C.c_float), ("reserved", C.c_uint32)]
+
+
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double)
 
 _hip = None
@@ -73,6 +77,16 @@ def hip_lib() -> C.CDLL:
             "ccd_loop_time": ([vp, f64p], C.c_int32),
             "ccd_copy_probe": ([vp, C.c_uint64, C.c_int32, f64p], C.c_int32),
             "ccd_set_plan": ([vp, C.POINTER(CPlan)], C.c_int32),
+            "ccd_batch_create": ([C.c_int32, C.POINTER(vp)], C.c_int32),
+            "ccd_batch_destroy": ([vp], C.c_int32),
+            "ccd_batch_last_error": ([vp], C.c_char_p),
+            "ccd_batch_add": ([vp, vp, C.POINTER(CParams)], C.c_int32),
+            "ccd_batch_set_plan": ([vp, C.POINTER(CBatchPlan)], C.c_int32),
+            "ccd_batch_get_plan": ([vp, C.POINTER(CBatchPlan)], C.c_int32),
+            "ccd_batch_tune": ([vp, f64p], C.c_int32),
+            "ccd_batch_run": ([vp, C.c_int32], C.c_int32),
+            "ccd_batch_time": ([vp, f64p], C.c_int32),
+            "ccd_batch_teams": ([vp, u32p, C.c_int32], C.c_int32),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -87,6 +101,8 @@ ABI_SYMBOLS = (
     "ccd_abi_version", "ccd_device_count", "ccd_params_default", "ccd_create", "ccd_destroy", "ccd_last_error",
     "ccd_upload", "ccd_upload_test", "ccd_set_factors", "ccd_run", "ccd_get_factors", "ccd_get_residual",
     "ccd_rmse", "ccd_iter_rmse", "ccd_kernel_stats", "ccd_loop_time", "ccd_copy_probe", "ccd_set_plan",
+    "ccd_batch_create", "ccd_batch_destroy", "ccd_batch_last_error", "ccd_batch_add", "ccd_batch_set_plan",
+    "ccd_batch_get_plan", "ccd_batch_tune", "ccd_batch_run", "ccd_batch_time", "ccd_batch_teams",
 )
 
 
@@ -264,6 +280,83 @@ class Solver:
         ms = C.c_double()
         self._check(self._lib.ccd_copy_probe(self._ctx, nbytes, reps, C.byref(ms)))
         return ms.value
+
+
+def make_params(k: int, lam: float, outer: int, inner: int, variant: int = VARIANT_TILED,
+                precision: int = PREC_FP32, order: int = ORDER_H_FIRST, zero_first: bool = True) -> CParams:
+    p = CParams()
+    hip_lib().ccd_params_default(C.byref(p))
+    p.k, p.lambda_, p.maxiter, p.maxinneriter = k, lam, outer, inner
+    p.variant, p.precision, p.order, p.zero_first = variant, precision, order, int(zero_first)
+    return p
+
+
+class Batch:
+    """Many independent factorizations solved by one persistent launch
+    (include/cdmf_hip.h, ccd_batch_*).  Each problem is a Solver with R
+    uploaded and factors set; results are read back from the Solvers."""
+
+    def __init__(self, device: int = 0):
+        self._lib = hip_lib()
+        self._b = C.c_void_p()
+        rc = self._lib.ccd_batch_create(device, C.byref(self._b))
+        if rc != 0:
+            raise CCDError(rc, "ccd_batch_create(%d) failed" % device)
+        self.solvers = []
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise CCDError(rc, self._lib.ccd_batch_last_error(self._b).decode())
+        return rc
+
+    def close(self) -> None:
+        if self._b:
+            self._lib.ccd_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, solver: Solver, k: int, lam: float, outer: int, inner: int, order: int = ORDER_H_FIRST,
+            zero_first: bool = True) -> int:
+        p = make_params(k, lam, outer, inner, VARIANT_TILED, PREC_FP32, order, zero_first)
+        idx = self._check(self._lib.ccd_batch_add(self._b, solver._ctx, C.byref(p)))
+        self.solvers.append(solver)
+        return idx
+
+    def set_plan(self, sigma: int = 16, blocks: int = 0, work_exponent: float = 1.0) -> None:
+        self._check(self._lib.ccd_batch_set_plan(self._b, C.byref(CBatchPlan(sigma, blocks, work_exponent, 0))))
+
+    def plan(self) -> dict:
+        pp = CBatchPlan()
+        self._check(self._lib.ccd_batch_get_plan(self._b, C.byref(pp)))
+        return dict(sigma=pp.sigma, blocks=pp.blocks, work_exponent=pp.work_exponent)
+
+    def tune(self) -> float:
+        ms = C.c_double()
+        self._check(self._lib.ccd_batch_tune(self._b, C.byref(ms)))
+        return ms.value
+
+    def run(self, use_graph: bool = False) -> float:
+        """One run of every problem; returns its device time in seconds."""
+        self._check(self._lib.ccd_batch_run(self._b, int(use_graph)))
+        s = C.c_double()
+        self._check(self._lib.ccd_batch_time(self._b, C.byref(s)))
+        return s.value
+
+    def teams(self) -> list:
+        buf = np.zeros(max(len(self.solvers), 1), dtype=np.uint32)
+        n = self._check(self._lib.ccd_batch_teams(self._b, N.ptr(buf, C.c_uint32), buf.size))
+        return [int(x) for x in buf[:n]]
 
 
 def cdmf_hip(R: Dataset, W: np.ndarray, H: np.ndarray, T: Dataset | None, param: Parameter,

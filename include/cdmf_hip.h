@@ -182,6 +182,50 @@ typedef struct ccd_plan_params {
 
 int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);
 
+/* ---------------------------------------------------------------- batched ---
+ * Many independent factorizations (hyperparameter sweeps, per-segment
+ * models) solved together by ONE persistent launch.  No counterpart in the
+ * reference, which solves one matrix per process (src/main.cpp:38-64).
+ *
+ * Each problem is an ordinary context (ccd_upload + ccd_set_factors done)
+ * with its own ccd_params (k, lambda, maxiter, maxinneriter, order,
+ * zero_first; FP32 only, maxinneriter >= 1).  The batch borrows the
+ * contexts: keep them alive and do not use them while the batch runs.
+ * After ccd_batch_run every context holds the result of its own ccd_run
+ * with the tiled sweeps (bit-identical, same sigma): read it with
+ * ccd_get_factors / ccd_get_residual / ccd_rmse, continue it with another
+ * batch run or a ccd_run.
+ */
+typedef struct ccd_batch ccd_batch;
+
+typedef struct ccd_batch_plan {
+    uint32_t sigma;       /* elements per lane of a tile: 8 or 16 (default 16) */
+    uint32_t blocks;      /* persistent blocks of 256 threads, a multiple of 8; 0 = all co-resident ones */
+    float work_exponent;  /* blocks are dealt to problems by (their work)^work_exponent (default 1) */
+    uint32_t reserved;
+} ccd_batch_plan;
+
+int32_t ccd_batch_create(int32_t device, ccd_batch** out);
+int32_t ccd_batch_destroy(ccd_batch* b);
+const char* ccd_batch_last_error(ccd_batch* b);
+/* returns the problem's index (>= 0) or an error code */
+int32_t ccd_batch_add(ccd_batch* b, ccd_ctx* ctx, const ccd_params* p);
+int32_t ccd_batch_set_plan(ccd_batch* b, const ccd_batch_plan* pp);
+int32_t ccd_batch_get_plan(ccd_batch* b, ccd_batch_plan* out);
+/* Autotuner: times short trial launches (about 1/64 of each problem's
+ * sweeps, no state change) for every sigma and work exponent and keeps the
+ * fastest plan; *best_ms = that trial's time. */
+int32_t ccd_batch_tune(ccd_batch* b, double* best_ms);
+/* One run of every problem: set-up fills and retiling, the persistent
+ * launch, write-back.  use_graph: capture that sequence in a HIP graph on
+ * first use and replay it while the plan and the problems' continued /
+ * zero_first state stay the same. */
+int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph);
+/* device time of the last run (events around the whole sequence) */
+int32_t ccd_batch_time(ccd_batch* b, double* seconds);
+/* blocks given to each problem by the last run; returns the problem count */
+int32_t ccd_batch_teams(ccd_batch* b, uint32_t* blocks, int32_t max_count);
+
 #ifdef __cplusplus
 }
 #endif
