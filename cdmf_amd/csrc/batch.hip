@@ -25,6 +25,7 @@ struct ccd_batch {
     ccd::BatchProblem* d_probs = nullptr;
     uint32_t* d_block_problem = nullptr;
     uint32_t* d_barriers = nullptr;
+    uint32_t* d_tickets = nullptr;  // 8 x 16 words
     int* d_status = nullptr;
     size_t cap_probs = 0, cap_blocks = 0;
     // captured run
@@ -117,40 +118,70 @@ double problem_work(const ccd_batch::Item& it) {
     return (double)it.p.maxiter * it.p.k * (2.0 * it.p.maxinneriter + 2.0) * (double)it.ctx->nnz;
 }
 
-// Blocks per problem: one each, then one at a time to the problem with the
-// largest work^e per block, never more than its tiles can use.
-std::vector<uint32_t> split_grid(const ccd_batch* b, uint32_t blocks, float expo, int sigma) {
+// Blocks per problem and their placement.  Sizes: one block each, then one
+// at a time to the problem with the largest work^e per block, never more
+// than its tiles can use nor than one XCD holds (blocks / 8).  Placement:
+// largest team first into the XCD with the most free blocks (shrunk if it
+// no longer fits), then each XCD's leftover blocks go to its own teams by
+// the same rule.  first[i]: the team's first virtual block id.
+struct Split {
+    std::vector<uint32_t> nb, first;
+};
+
+Split split_grid(const ccd_batch* b, uint32_t blocks, float expo, int sigma) {
     const size_t np = b->items.size();
-    std::vector<uint32_t> nb(np, 1u), cap(np);
+    const uint32_t C = blocks / 8;
+    std::vector<uint32_t> nb(np, 1u), cap(np), xcd(np);
     std::vector<double> w(np);
     for (size_t i = 0; i < np; ++i) {
         ccd_ctx* c = b->items[i].ctx;
         const uint64_t te = 64ull * (uint64_t)sigma;
         const uint64_t tiles = (c->nnz + te - 1) / te;
         cap[i] = (uint32_t)std::max<uint64_t>(1, (tiles + ccd::BATCH_BLOCK / 64 - 1) / (ccd::BATCH_BLOCK / 64));
+        cap[i] = std::min(cap[i], C);
         w[i] = std::pow(std::max(problem_work(b->items[i]), 1.0), (double)expo);
     }
-    for (uint32_t left = blocks - (uint32_t)np; left > 0; --left) {
-        size_t best = np;
-        double bw = -1.0;
-        for (size_t i = 0; i < np; ++i)
-            if (nb[i] < cap[i] && w[i] / nb[i] > bw) {
-                bw = w[i] / nb[i];
-                best = i;
-            }
-        if (best == np) break;
-        ++nb[best];
+    auto grow = [&](uint32_t left, int only_xcd) {
+        for (; left > 0; --left) {
+            size_t best = np;
+            double bw = -1.0;
+            for (size_t i = 0; i < np; ++i)
+                if ((only_xcd < 0 || (int)xcd[i] == only_xcd) && nb[i] < cap[i] && w[i] / nb[i] > bw) {
+                    bw = w[i] / nb[i];
+                    best = i;
+                }
+            if (best == np) return left;
+            ++nb[best];
+        }
+        return 0u;
+    };
+    grow(blocks - (uint32_t)np, -1);
+    std::vector<size_t> order(np);
+    for (size_t i = 0; i < np; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return nb[a] > nb[c]; });
+    std::vector<uint32_t> freeb(8, C);
+    for (size_t i : order) {
+        const uint32_t x = (uint32_t)(std::max_element(freeb.begin(), freeb.end()) - freeb.begin());
+        nb[i] = std::max(1u, std::min(nb[i], freeb[x]));  // np <= blocks keeps freeb[x] >= 1 here
+        xcd[i] = x;
+        freeb[x] -= nb[i];
     }
-    return nb;
+    for (uint32_t x = 0; x < 8; ++x) freeb[x] = grow(freeb[x], (int)x);
+    Split sp{nb, std::vector<uint32_t>(np)};
+    std::vector<uint32_t> used(8, 0);
+    for (size_t i = 0; i < np; ++i) {
+        sp.first[i] = xcd[i] * C + used[xcd[i]];
+        used[xcd[i]] += nb[i];
+    }
+    return sp;
 }
 
 // Upload the launch description (trial: the autotuner's short run).
-int describe(ccd_batch* b, int sigma, uint32_t blocks, const std::vector<uint32_t>& nb, bool trial,
-             ccd::BatchArgs* args) {
+int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool trial, ccd::BatchArgs* args) {
+    const std::vector<uint32_t>& nb = sp.nb;
     const size_t np = b->items.size();
     std::vector<ccd::BatchProblem> probs(np);
     std::vector<uint32_t> owner(blocks, 0xffffffffu);
-    uint32_t first = 0;
     for (size_t i = 0; i < np; ++i) {
         const ccd_batch::Item& it = b->items[i];
         ccd_ctx* c = it.ctx;
@@ -166,14 +197,16 @@ int describe(ccd_batch* b, int sigma, uint32_t blocks, const std::vector<uint32_
         P.T = trial ? 0u : (uint32_t)it.p.maxiter;
         P.Tin = (uint32_t)it.p.maxinneriter;
         P.continued = c->continued ? 1u : 0u;
-        P.first = first;
+        P.first = sp.first[i];
         P.nblocks = nb[i];
         P.lambda = it.p.lambda;
         // a trial runs about 1/64 of each problem's sweeps, so that the
         // slowest team sets its time as it sets the real run's
         P.trial_reps = (uint32_t)std::max(1.0, std::round((double)it.p.k * it.p.maxiter * it.p.maxinneriter / 64.0));
-        for (uint32_t v = 0; v < nb[i]; ++v) owner[first + v] = (uint32_t)i;
-        first += nb[i];
+        for (uint32_t v = 0; v < nb[i]; ++v) {
+            if (owner[sp.first[i] + v] != 0xffffffffu) return berr(b, CCD_ERR_INVALID, "internal: overlapping teams");
+            owner[sp.first[i] + v] = (uint32_t)i;
+        }
     }
     if (np > b->cap_probs) {
         ccdi::dfree(b->d_probs);
@@ -197,6 +230,7 @@ int describe(ccd_batch* b, int sigma, uint32_t blocks, const std::vector<uint32_
     args->nprob = (uint32_t)np;
     args->nblocks = blocks;
     args->barriers = b->d_barriers;
+    args->xcd_tickets = b->d_tickets;
     args->status = b->d_status;
     args->wait_ticks = 100000000ull * 4;  // 4 s of the 100 MHz wall clock
     args->trial = trial ? 1u : 0u;
@@ -209,6 +243,7 @@ int enqueue_run(ccd_batch* b, int sigma, const ccd::BatchArgs& a, bool trial) {
     hipStream_t s = b->stream;
     BATCH_TRY(b, ccd::launch_fill(reinterpret_cast<float*>(b->d_barriers), 0.0f, a.nprob, s));
     BATCH_TRY(b, ccd::launch_fill(reinterpret_cast<float*>(b->d_status), 0.0f, 1, s));
+    BATCH_TRY(b, ccd::launch_fill(reinterpret_cast<float*>(b->d_tickets), 0.0f, 8 * 16, s));
     for (const ccd_batch::Item& it : b->items) {
         ccd_ctx* c = it.ctx;
         ccdi::DevLayout& F = first_layout(c, it.p);
@@ -252,15 +287,16 @@ int grid_blocks(ccd_batch* b, int sigma, uint32_t want, uint32_t* out) {
 int check_status(ccd_batch* b) {
     int st = 0;
     BATCH_TRY(b, hipMemcpy(&st, b->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st == 2) return berr(b, CCD_ERR_HIP, "blocks were not spread evenly over 8 XCDs: team barriers are unsafe here");
     if (st) return berr(b, CCD_ERR_HIP, "a team barrier wait ran out (blocks not co-resident?)");
     return CCD_OK;
 }
 
 int timed_trial(ccd_batch* b, int sigma, uint32_t blocks, float expo, double* ms) {
     if (int rc = prepare(b, sigma)) return rc;
-    const std::vector<uint32_t> nb = split_grid(b, blocks, expo, sigma);
+    const Split sp = split_grid(b, blocks, expo, sigma);
     ccd::BatchArgs a{};
-    if (int rc = describe(b, sigma, blocks, nb, true, &a)) return rc;
+    if (int rc = describe(b, sigma, blocks, sp, true, &a)) return rc;
     if (int rc = enqueue_run(b, sigma, a, true)) return rc;  // warm
     BATCH_TRY(b, hipEventRecord(b->ev_a, b->stream));
     if (int rc = enqueue_run(b, sigma, a, true)) return rc;
@@ -288,7 +324,7 @@ int32_t ccd_batch_create(int32_t device, ccd_batch** out) {
     b->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess ||
-        ccdi::dalloc(&b->d_status, 1) != hipSuccess) {
+        ccdi::dalloc(&b->d_status, 1) != hipSuccess || ccdi::dalloc(&b->d_tickets, 8 * 16) != hipSuccess) {
         delete b;
         return CCD_ERR_HIP;
     }
@@ -307,6 +343,7 @@ int32_t ccd_batch_destroy(ccd_batch* b) {
     ccdi::dfree(b->d_block_problem);
     ccdi::dfree(b->d_barriers);
     ccdi::dfree(b->d_status);
+    ccdi::dfree(b->d_tickets);
     if (b->ev_a) (void)hipEventDestroy(b->ev_a);
     if (b->ev_b) (void)hipEventDestroy(b->ev_b);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -325,8 +362,6 @@ int32_t ccd_batch_add(ccd_batch* b, ccd_ctx* c, const ccd_params* p) {
         return berr(b, CCD_ERR_INVALID, "batched problems need maxiter >= 0, maxinneriter >= 1, lambda >= 0 finite");
     if (p->precision != CCD_PREC_FP32) return berr(b, CCD_ERR_UNSUPPORTED, "batched problems run in FP32");
     if (p->order != CCD_ORDER_H_FIRST && p->order != CCD_ORDER_W_FIRST) return berr(b, CCD_ERR_INVALID, "bad order");
-    if (p->zero_first && c->continued)
-        return berr(b, CCD_ERR_INVALID, "zero_first on a context that continues a previous run");
     for (const ccd_batch::Item& it : b->items)
         if (it.ctx == c) return berr(b, CCD_ERR_INVALID, "context added twice");
     b->items.push_back({c, *p});
@@ -386,10 +421,10 @@ int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph) {
     if (int rc = prepare(b, sigma)) return rc;
     uint32_t blocks = 0;
     if (int rc = grid_blocks(b, sigma, b->plan.blocks, &blocks)) return rc;
-    const std::vector<uint32_t> nb = split_grid(b, blocks, b->plan.work_exponent, sigma);
+    const Split sp = split_grid(b, blocks, b->plan.work_exponent, sigma);
     ccd::BatchArgs a{};
-    if (int rc = describe(b, sigma, blocks, nb, false, &a)) return rc;
-    b->team = nb;
+    if (int rc = describe(b, sigma, blocks, sp, false, &a)) return rc;
+    b->team = sp.nb;
 
     // the captured sequence depends on the plan, the grid and each problem's
     // continued / zero_first state

@@ -17,10 +17,10 @@
 // wait that runs out sets *status and the whole team leaves the kernel, so
 // the grid always drains.
 //
-// Block placement: workgroups are dealt to the 8 XCDs round-robin by block
-// index, so virtual block id v = (b % 8) * (B / 8) + b / 8 puts consecutive
-// virtual ids on one XCD; teams are contiguous virtual ranges and a small
-// problem's gathered vectors stay in one XCD's L2.
+// Block placement: a block takes virtual id x * B/8 + (its ticket on XCD
+// x), so the virtual ids [x B/8, (x+1) B/8) run on XCD x.  The host packs
+// every team into one such range: a team shares one L2 (cheap barriers, see
+// team_sync) and keeps its problem's gathered vectors there.
 #include "tile_body.h"
 
 namespace ccd {
@@ -37,27 +37,34 @@ struct Team {
     uint64_t wait_ticks;
 };
 
-// Barrier of the blocks of one team.  false: the wait ran out (status set);
-// the caller returns.
+// Barrier of the blocks of one team; all of a team's blocks run on one XCD
+// and share its L2.  Release: every wave waits until its stores are
+// acknowledged (s_waitcnt vmcnt(0)), then one arrival per block is counted
+// with an L2 atomic.  Acquire: after the count is complete, an agent-scope
+// acquire fence drops the CU's stale cached copies, so the loads that follow
+// see the other blocks' data.  (No L2 write-back is needed inside one XCD;
+// an agent-scope release fence, which does one, costs 30-40 us a barrier.)
+// false: the wait ran out (status set); the caller returns.
 __device__ __forceinline__ bool team_sync(Team& t) {
     __shared__ int s_ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (t.nb == 1) return true;
     t.target += t.nb;
     if (threadIdx.x == 0) {
         int ok = 1;
-        __threadfence();
-        __hip_atomic_fetch_add(t.cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(t.cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const long long t0 = wall_clock64();
-        while (__hip_atomic_load(t.cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < t.target) {
-            __builtin_amdgcn_s_sleep(2);
+        while (__hip_atomic_load(t.cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t.target) {
+            __builtin_amdgcn_s_sleep(1);
             if ((uint64_t)(wall_clock64() - t0) > t.wait_ticks) {
                 __hip_atomic_store(t.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
@@ -144,7 +151,26 @@ __device__ __forceinline__ void tiles(const BatchLayout* L, const PassVecs& v, b
 
 template <int SIGMA>
 __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
-    const uint32_t v = (blockIdx.x % 8u) * (ba.nblocks / 8u) + blockIdx.x / 8u;
+    // Virtual block id: the XCD this block runs on (hardware XCC id
+    // register) and a ticket among that XCD's blocks.  Workgroups are dealt
+    // to the 8 XCDs round-robin, so each XCD receives nblocks / 8 of them
+    // (the starting XCD depends on earlier launches, hence the tickets).
+    __shared__ uint32_t s_v;
+    if (threadIdx.x == 0) {
+        const uint32_t C = ba.nblocks / 8u;
+        const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // HW_REG_XCC_ID[3:0]
+        uint32_t v = 0xffffffffu;
+        if (xcc < 8u) {
+            const uint32_t slot = __hip_atomic_fetch_add(ba.xcd_tickets + 16u * xcc, 1u, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            if (slot < C) v = xcc * C + slot;
+        }
+        if (v == 0xffffffffu) __hip_atomic_store(ba.status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_v = v;
+    }
+    __syncthreads();
+    const uint32_t v = s_v;
+    if (v == 0xffffffffu) return;
     const uint32_t p = ba.block_problem[v];
     if (p >= ba.nprob) return;
     const BatchProblem& P = ba.probs[p];

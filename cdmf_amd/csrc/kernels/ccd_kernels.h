@@ -318,6 +318,7 @@ struct BatchArgs {
     uint32_t nprob;
     uint32_t nblocks;               // grid size (a multiple of 8)
     uint32_t* barriers;             // one counter per problem, zero at launch
+    uint32_t* xcd_tickets;          // 8 counters, 64 bytes apart, zero at launch
     int* status;                    // 0; 1 after a barrier wait ran out
     uint64_t wait_ticks;            // bound of one barrier wait in wall_clock64() ticks
     uint32_t trial;                 // autotuner trial: trial_reps plain sweep pairs per problem, nothing else
