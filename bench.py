@@ -96,6 +96,9 @@ def main():
                     help="the batched workload instead: 64 independent problems in one persistent launch, "
                          "against 64 sequential single-problem runs")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="shrink the batched problems (quick checks)")
+    ap.add_argument("--foldin", action="store_true",
+                    help="the held-out-users fold-in study on the large shape (plain vs learned warm start)")
+    ap.add_argument("--foldin-users", type=int, default=0, help="rows of the fold-in study's matrix (0: 480,189)")
     ap.add_argument("--probe", action="store_true",
                     help="run a 256 MiB stream-copy kernel first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -112,6 +115,9 @@ def main():
         sys.exit("bench.py: no HIP device visible")
     if args.batch:
         bench_batch(args)
+        return
+    if args.foldin:
+        bench_foldin(args)
         return
 
     if args.small:
@@ -301,6 +307,43 @@ def bench_batch(args):
         "max_test_rmse_diff_vs_sequential": worst,
         "teams": teams,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_foldin(args):
+    """Held-out-users fold-in on the 480,189 x 17,770 shape, k = 40: 10 % of
+    the users are held out, CCD++ factorizes the rest (10 outer x 5 inner),
+    the MLP (82 -> 256 -> 256 -> 40, GELU) is trained on the kept users'
+    (features, w_i) pairs and run with bfloat16 weights, and the held-out
+    users are folded in from zero and from the warm start.  Both rating
+    models of the shape: the independent 1..5 ratings and the planted
+    rank-8 ratings (same frequencies)."""
+    from cdmf_amd import data as D
+    from cdmf_amd import foldin as F
+
+    out = {"metric": "fold-in sweeps / device time to the plain fold-in's test RMSE (warm start vs zero start)",
+           "n_gpus": 1, "dtype": "fp32 (MLP weights bf16)", "studies": {}}
+    for name, kind in (("independent ratings", 1), ("planted rank-8 ratings", 3)):
+        spec = dict(D.NETFLIX, rating_kind=kind)
+        if args.foldin_users:
+            scale = args.foldin_users / spec["m"]
+            spec.update(m=args.foldin_users, nnz_train=int(spec["nnz_train"] * scale),
+                        nnz_test=int(spec["nnz_test"] * scale))
+        t = time.perf_counter()
+        ds = D.gen_powerlaw(**spec)
+        log("%s: generated %dx%d nnz=%d in %.1fs" % (name, ds.m, ds.n, ds.nnz, time.perf_counter() - t))
+        cfg = D.FOLDIN_SOLVE
+        r = F.experiment(ds, cfg.k, cfg.lam, cfg.outer, cfg.inner, sweeps=8, epochs=20, log=log)
+        rec = dict(users=ds.m, held_out=len(r.plain_rmse) and None, target_rmse=r.target,
+                   plain_sweeps=r.plain_sweeps, warm_sweeps_to_target=r.warm_sweeps,
+                   plain_time_to_target_s=r.plain_time[r.plain_sweeps],
+                   warm_time_to_target_s=r.warm_time[r.warm_sweeps] if r.warm_sweeps >= 0 else None,
+                   plain_rmse=r.plain_rmse, warm_rmse=r.warm_rmse, plain_time_s=r.plain_time, warm_time_s=r.warm_time,
+                   objective_star=r.obj_star, plain_objective=r.plain_obj, warm_objective=r.warm_obj,
+                   plain_sweeps_to_objective=r.plain_sweeps_obj, warm_sweeps_to_objective=r.warm_sweeps_obj,
+                   mlp_train_loss=r.train_loss, timings=r.timings)
+        rec.pop("held_out")
+        out["studies"][name] = rec
     print(json.dumps(out), flush=True)
 
 

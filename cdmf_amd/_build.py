@@ -34,8 +34,11 @@ HIP_SOURCES = [
     "kernels/ccd_kernels.hip",
     "kernels/tiled_kernels.hip",
     "kernels/batch_kernels.hip",
+    "kernels/foldin_kernels.hip",
+    "kernels/mlp_kernels.hip",
     "ccd_abi.hip",
     "batch.hip",
+    "foldin.hip",
 ]
 HOST_SOURCES = [
     "host/tools.cpp",

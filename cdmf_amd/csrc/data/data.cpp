@@ -12,6 +12,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <memory>
 #include <unordered_set>
 #include <vector>
 
@@ -72,7 +73,7 @@ struct RatingTable {
     std::vector<float> value;
     explicit RatingTable(int kind) {
         std::vector<double> p;
-        if (kind == CDMF_RATING_NETFLIX_1_5) {
+        if (kind == CDMF_RATING_NETFLIX_1_5 || kind == CDMF_RATING_PLANTED_1_5) {
             p = {0.05, 0.10, 0.29, 0.34, 0.22};
             for (int r = 1; r <= 5; ++r) value.push_back((float)r);
         } else if (kind == CDMF_RATING_HALF_STAR) {
@@ -87,11 +88,46 @@ struct RatingTable {
         for (double q : p) { c += q / s; cdf.push_back(c); }
         cdf.back() = 1.0;
     }
-    inline float draw(Xoshiro& g) const {
-        double u = g.uniform();
+    inline float draw(Xoshiro& g) const { return at(g.uniform()); }
+    // the rating whose cumulative frequency band holds u in [0, 1)
+    inline float at(double u) const {
         size_t i = 0;
         while (i + 1 < cdf.size() && u >= cdf[i]) ++i;
         return value[i];
+    }
+};
+
+// Hidden model of CDMF_RATING_PLANTED_1_5: per row a_i, u_i (PLANT_K values),
+// per column b_j, v_j; score = a_i + b_j + u_i.v_j + e, every term Gaussian,
+// so score / sd is close to N(0, 1) and Phi(score / sd) close to uniform.
+constexpr int PLANT_K = 8;
+constexpr double PLANT_SA = 0.5, PLANT_SB = 0.5, PLANT_SU = 0.5, PLANT_SE = 0.5;
+
+inline double gauss(Xoshiro& g) {  // Box-Muller
+    const double u1 = std::max(g.uniform(), 1e-300), u2 = g.uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+struct Planted {
+    std::vector<float> a, u, b, v;
+    double sd = 1.0;
+    Planted(int64_t m, int64_t n, uint64_t seed) : a(m), u((size_t)m * PLANT_K), b(n), v((size_t)n * PLANT_K) {
+        for (int64_t i = 0; i < m; ++i) {
+            Xoshiro g(stream_seed(seed, 0x7a11, (uint64_t)i));
+            a[i] = (float)(PLANT_SA * gauss(g));
+            for (int q = 0; q < PLANT_K; ++q) u[(size_t)i * PLANT_K + q] = (float)(PLANT_SU * gauss(g));
+        }
+        for (int64_t j = 0; j < n; ++j) {
+            Xoshiro g(stream_seed(seed, 0x7b22, (uint64_t)j));
+            b[j] = (float)(PLANT_SB * gauss(g));
+            for (int q = 0; q < PLANT_K; ++q) v[(size_t)j * PLANT_K + q] = (float)(PLANT_SU * gauss(g));
+        }
+        sd = std::sqrt(PLANT_SA * PLANT_SA + PLANT_SB * PLANT_SB + PLANT_K * std::pow(PLANT_SU, 4) + PLANT_SE * PLANT_SE);
+    }
+    float rating(const RatingTable& rt, int64_t i, uint32_t j, Xoshiro& g) const {
+        double sc = (double)a[i] + (double)b[j] + PLANT_SE * gauss(g);
+        for (int q = 0; q < PLANT_K; ++q) sc += (double)u[(size_t)i * PLANT_K + q] * v[(size_t)j * PLANT_K + q];
+        return rt.at(0.5 * std::erfc(-(sc / sd) / std::sqrt(2.0)));
     }
 };
 
@@ -286,6 +322,8 @@ int cdmf_gen_uniform(int64_t m, int64_t n, uint64_t nnz_train, uint64_t nnz_test
     const uint64_t total = nnz_train + nnz_test;
     const uint64_t cells = (uint64_t)m * (uint64_t)n;
     if (total > cells) return fail(CDMF_ERR_INVALID, "more entries than matrix cells");
+    if (rating_kind == CDMF_RATING_PLANTED_1_5)
+        return fail(CDMF_ERR_INVALID, "planted ratings come from the power-law generator only");
     // Floyd's algorithm: `total` distinct cells out of `cells`.
     Xoshiro g(stream_seed(seed, 0x0f10, 0));
     std::unordered_set<uint64_t> seen;
@@ -385,6 +423,8 @@ int cdmf_gen_powerlaw(const cdmf_gen_spec* spec, cdmf_dataset* out) {
     for (int64_t r = 0; r < n; ++r) w[perm[r]] = std::pow((double)(r + 1), -spec->col_exponent);
     Alias alias(w);
     RatingTable rt(spec->rating_kind);
+    std::unique_ptr<Planted> plant;
+    if (spec->rating_kind == CDMF_RATING_PLANTED_1_5) plant.reset(new Planted(m, n, seed));
 
     std::vector<uint64_t> row_off(m + 1, 0);
     for (int64_t i = 0; i < m; ++i) row_off[i + 1] = row_off[i] + deg[i];
@@ -424,7 +464,10 @@ int cdmf_gen_powerlaw(const cdmf_gen_spec* spec, cdmf_dataset* out) {
             }
             std::sort(c, c + d);
             Xoshiro gr(stream_seed(seed, 0x4a7e, (uint64_t)i));
-            for (uint32_t q = 0; q < d; ++q) vals[row_off[i] + q] = rt.draw(gr);
+            if (plant)
+                for (uint32_t q = 0; q < d; ++q) vals[row_off[i] + q] = plant->rating(rt, i, c[q], gr);
+            else
+                for (uint32_t q = 0; q < d; ++q) vals[row_off[i] + q] = rt.draw(gr);
         }
     }
     std::vector<uint8_t> is_test;

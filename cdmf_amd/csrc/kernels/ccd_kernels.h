@@ -328,4 +328,70 @@ struct BatchArgs {
 hipError_t launch_batch(const BatchArgs& a, int sigma, hipStream_t s);
 int batch_blocks_per_cu(int sigma);
 
+// ---------------------------------------------------------------- fold-in ---
+// Rows (CSR, the new users' ratings) folded in against fixed item factors:
+// H [k][n] rank-major for the sweeps, HT [n][k] for the features.
+constexpr int FOLD_KMAX = 128;
+struct FoldArgs {
+    const uint32_t* ptr;  // rows + 1
+    const uint32_t* idx;
+    const float* val;
+    const float* H;
+    const float* HT;
+    uint32_t rows;
+    uint32_t k;
+    uint32_t n;
+    int sweeps;
+    float lambda;
+};
+// Rows longer than FOLD_SHORT ratings are listed in long_rows[0 .. nlong)
+// and get a block each.
+constexpr uint32_t FOLD_SHORT = 1024;
+// F [rows][2k + 2]: rating-weighted mean and variance of the rated items' H
+// columns, log(1 + degree), mean rating (FP64 sums)
+hipError_t launch_fold_features(const FoldArgs& a, const uint32_t* long_rows, uint32_t nlong, float* F,
+                                hipStream_t s);
+// W [rows][k] after a.sweeps sweeps from W0 ([rows][k], null = zeros); rbuf:
+// nnz floats of scratch (used by rows above 16384 ratings)
+hipError_t launch_fold_sweeps(const FoldArgs& a, const uint32_t* long_rows, uint32_t nlong, const float* W0, float* W,
+                              float* rbuf, hipStream_t s);
+hipError_t launch_transpose(const float* src, float* dst, uint32_t rows, uint32_t cols, hipStream_t s);
+
+// -------------------------------------------------------------------- MLP ---
+// C(i, j) = sum_p A(i, p) B(p, j) for i < M, j < N, p < K with
+// A(i, p) = A[i sai + p sap] (FP32), B(p, j) = B[p sbp + j sbj] (FP32 or
+// bfloat16); epilogues: GEMM_STORE C = acc (+ bias[j]); GEMM_BIAS_GELU
+// C = acc + bias[j], C2 = GELU(C); GEMM_GELU_GRAD C = acc * GELU'(aux)
+// (aux, C, C2 row-major with leading dimension ldc).
+constexpr int GEMM_STORE = 0, GEMM_BIAS_GELU = 1, GEMM_GELU_GRAD = 2;
+struct GemmArgs {
+    const float* A;
+    uint64_t sai, sap, sbp, sbj;
+    uint32_t M, N, K, ldc;
+    const void* bias;  // same element type as B
+    float* C;
+    float* C2;
+    const float* aux;
+    float* part;       // optional workspace of part_cap floats: a plain product (GEMM_STORE, no
+    uint64_t part_cap; // bias) with few output tiles splits K over slices, summed in slice order
+};
+hipError_t launch_gemm(const GemmArgs& g, const void* B, bool bf16_b, int epi, hipStream_t s);
+hipError_t launch_colsum(const float* dZ, uint32_t M, uint32_t N, float* out, hipStream_t s);
+// dY = 2 (Y - T) / count; *out = mean (Y - T)^2 (FP64, fixed order); part: MSE_PARTS doubles
+constexpr uint32_t MSE_PARTS = 1024;
+hipError_t launch_mse(const float* Y, const float* T, uint64_t count, float* dY, double* part, double* out,
+                      hipStream_t s);
+struct AdamArgs {
+    float* param;
+    const float* grad;
+    float* m;
+    float* v;
+    uint64_t count;
+    float lr, beta1, beta2, eps, bc1, bc2;  // bc = 1 - beta^step
+};
+hipError_t launch_adam(const AdamArgs& a, hipStream_t s);
+// Y[i][j] = (X[r][j] - sub[j]) * mul[j], r = rows ? rows[i] : i (sub null: plain copy / gather)
+hipError_t launch_affine_rows(const float* X, const uint32_t* rows, uint32_t M, uint32_t N, const float* sub,
+                              const float* mul, float* Y, hipStream_t s);
+
 }  // namespace ccd

@@ -40,6 +40,12 @@ NETFLIX = dict(m=480189, n=17770, nnz_train=99072112, nnz_test=1408395, max_row_
                row_alpha=0.0, col_exponent=1.0, rating_kind=1, seed=1)
 NETFLIX_SOLVE = SolveConfig(k=40, lam=0.05, outer=10, inner=5)
 
+# The held-out-users fold-in study on the large shape: the same matrix, and
+# the same shape with ratings planted from a hidden rank-8 model (same 1..5
+# frequencies), on which a warm start has something to learn.
+NETFLIX_PLANTED = dict(NETFLIX, rating_kind=3)
+FOLDIN_SOLVE = NETFLIX_SOLVE
+
 # The precision-study case: 138,493 x 26,744 with 20.0M ratings (10 % test),
 # power-law degrees with exponent 1.1 on the column side, half-star ratings.
 ML20M = dict(m=138493, n=26744, nnz_train=18000237, nnz_test=2000026, max_row_degree=9254,
@@ -120,6 +126,10 @@ def small() -> Dataset:
 
 def netflix(threads: int = 0) -> Dataset:
     return gen_powerlaw(threads=threads, **NETFLIX)
+
+
+def netflix_planted(threads: int = 0) -> Dataset:
+    return gen_powerlaw(threads=threads, **NETFLIX_PLANTED)
 
 
 def ml20m(threads: int = 0) -> Dataset:

@@ -47,7 +47,11 @@ typedef struct cdmf_dataset {
 enum cdmf_rating_kind {
     CDMF_RATING_UNIFORM_1_5 = 0,  /* integers 1..5, equal probability */
     CDMF_RATING_NETFLIX_1_5 = 1,  /* integers 1..5 with P = .05 .10 .29 .34 .22 */
-    CDMF_RATING_HALF_STAR = 2     /* 0.5..5.0 in steps of 0.5, weighted toward 3.5-4.0 */
+    CDMF_RATING_HALF_STAR = 2,    /* 0.5..5.0 in steps of 0.5, weighted toward 3.5-4.0 */
+    CDMF_RATING_PLANTED_1_5 = 3   /* integers 1..5 with the NETFLIX_1_5 frequencies, ordered by a hidden
+                                     rank-8 model: score = a_i + b_j + u_i.v_j + noise (Gaussian, seeded per
+                                     row, column and entry), rating = the score's normal quantile mapped
+                                     through the 1..5 frequencies (power-law generator only) */
 };
 
 /* Power-law synthetic ratings matrix.

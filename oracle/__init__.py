@@ -9,6 +9,9 @@ Function -> reference it restates (see ccd_oracle.c for the details):
   initial_col src/tools.cpp:349-357      dual_spmv   src/detail/kcode/csr5_spmv.cl:134-147
   ccd_f64 / ccd_bf16 / objective / residual_f64 / sweep_f64: no reference
   counterpart; pinned by the mathematical checks in tests/test_oracle.py.
+  fold_rows / fold_features / mlp_forward / mlp_backward (foldin_oracle.c):
+  the fold-in extension, no reference counterpart; pinned by
+  tests/test_oracle_foldin.py (ridge solve, finite differences).
 """
 from __future__ import annotations
 
@@ -63,6 +66,14 @@ def lib() -> C.CDLL:
         L.or_objective.restype = dbl
         L.or_residual_f64.argtypes = [i64, i64, u32p, u32p, f32p, f32p, f32p, u32, f64p]
         L.or_residual_f64.restype = None
+        L.or_fold_rows.argtypes = [i64, u32p, u32p, f32p, u32, i64, f32p, flt, i32, f32p, i32]
+        L.or_fold_rows.restype = i32
+        L.or_fold_features.argtypes = [i64, u32p, u32p, f32p, u32, i64, f32p, f32p]
+        L.or_fold_features.restype = None
+        L.or_mlp_forward.argtypes = [i64, i32, i32, i32, f64p, f64p, f64p]
+        L.or_mlp_forward.restype = None
+        L.or_mlp_backward.argtypes = [i64, i32, i32, i32, f64p, f64p, f64p, f64p]
+        L.or_mlp_backward.restype = dbl
         _lib = L
     return _lib
 
@@ -170,3 +181,52 @@ def residual_f64(ds, W, H):
     L.or_residual_f64(ds.m, ds.n, _p(cp, C.c_uint32), _p(ci, C.c_uint32), _p(v, C.c_float), _p(W, C.c_float),
                       _p(H, C.c_float), W.shape[0], _p(out, C.c_double))
     return out
+
+
+def fold_rows(ptr, idx, val, H, lam, sweeps, W0=None, threads=0):
+    """CCD++ fold-in of CSR rows against fixed item factors H [k][n]:
+    returns W [rows][k] (float32) after `sweeps` sweeps from W0 (zeros)."""
+    L = lib()
+    ptr, idx, val, H = _c(ptr, np.uint32), _c(idx, np.uint32), _c(val, np.float32), _c(H, np.float32)
+    k, n = H.shape
+    rows = ptr.size - 1
+    W = np.zeros((rows, k), np.float32) if W0 is None else np.array(W0, dtype=np.float32, copy=True)
+    rc = L.or_fold_rows(rows, _p(ptr, C.c_uint32), _p(idx, C.c_uint32), _p(val, C.c_float), k, n, _p(H, C.c_float),
+                        lam, sweeps, _p(W, C.c_float), threads)
+    if rc:
+        raise MemoryError("or_fold_rows")
+    return W
+
+
+def fold_features(ptr, idx, val, H):
+    """Fold-in features [rows][2k + 2]: weighted mean and variance of the rated
+    items' H columns, log(1 + degree), mean rating."""
+    L = lib()
+    ptr, idx, val, H = _c(ptr, np.uint32), _c(idx, np.uint32), _c(val, np.float32), _c(H, np.float32)
+    k, n = H.shape
+    rows = ptr.size - 1
+    F = np.empty((rows, 2 * k + 2), np.float32)
+    L.or_fold_features(rows, _p(ptr, C.c_uint32), _p(idx, C.c_uint32), _p(val, C.c_float), k, n, _p(H, C.c_float),
+                       _p(F, C.c_float))
+    return F
+
+
+def mlp_forward(params, X, dims):
+    """FP64 MLP forward; params flat (W1, b1, W2, b2, W3, b3), dims (in, hid, out)."""
+    L = lib()
+    P, X = _c(params, np.float64), _c(X, np.float64)
+    nin, hid, out = dims
+    Y = np.empty((X.shape[0], out))
+    L.or_mlp_forward(X.shape[0], nin, hid, out, _p(P, C.c_double), _p(X, C.c_double), _p(Y, C.c_double))
+    return Y
+
+
+def mlp_backward(params, X, T, dims):
+    """FP64 (loss, gradient) of mean((MLP(X) - T)^2) w.r.t. the flat parameters."""
+    L = lib()
+    P, X, T = _c(params, np.float64), _c(X, np.float64), _c(T, np.float64)
+    nin, hid, out = dims
+    G = np.empty_like(P)
+    loss = L.or_mlp_backward(X.shape[0], nin, hid, out, _p(P, C.c_double), _p(X, C.c_double), _p(T, C.c_double),
+                             _p(G, C.c_double))
+    return loss, G

@@ -15,7 +15,8 @@ def _declared(header):
     return sorted(set(re.findall(r"\b((?:ccd|cdmf)_[a-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,lib", [("cdmf_hip.h", "libcdmf_hip.so"), ("cdmf_data.h", "libcdmf_data.so")])
+@pytest.mark.parametrize("header,lib", [("cdmf_hip.h", "libcdmf_hip.so"), ("cdmf_data.h", "libcdmf_data.so"),
+                                        ("cdmf_foldin.h", "libcdmf_hip.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = _declared(header)
     assert len(names) >= 8
@@ -25,8 +26,10 @@ def test_library_exports_every_declared_symbol(header, lib):
 
 
 def test_python_binding_covers_the_header():
-    from cdmf_amd import solver
+    from cdmf_amd import foldin, solver
     assert sorted(solver.ABI_SYMBOLS) == _declared("cdmf_hip.h")
+    assert sorted(foldin.FOLDIN_SYMBOLS) == _declared("cdmf_foldin.h")
+    foldin._lib()  # every declared signature binds
 
 
 def test_no_device_fails_cleanly():
