@@ -122,6 +122,12 @@ HostPlan make_plan(const uint32_t* ptr, int64_t rows, const ccd_plan_params& pp)
     return P;
 }
 
+void drop_graph(ccd_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_key.clear();
+}
+
 // ----------------------------------------------------------- profiling --
 hipEvent_t take_event(ccd_ctx* c) {
     if (c->pool_used == c->pool.size()) {
@@ -132,24 +138,35 @@ hipEvent_t take_event(ccd_ctx* c) {
     return c->pool[c->pool_used++];
 }
 
+// Times the launches made in its scope as one record of `kind` when that
+// kind is profiled.  Default: event records before and after.  bound: the
+// events are handed to a single launch (ev()) that binds them to its own
+// dispatch, so nothing is added between kernels.
 struct Timed {
     ccd_ctx* c;
     int kind;
-    hipEvent_t a = nullptr;
-    Timed(ccd_ctx* ctx, int k) : c(ctx), kind(k) {
-        if (c->prof) {
-            a = take_event(c);
-            if (a) (void)hipEventRecord(a, c->stream);
+    bool bound;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(ccd_ctx* ctx, int k, bool bind = false) : c(ctx), kind(k), bound(bind) {
+        if (!c->prof || (k >= 0 && c->prof_kinds && !((c->prof_kinds >> k) & 1u))) return;
+        a = take_event(c);
+        b = take_event(c);
+        if (!a || !b) {
+            a = b = nullptr;
+            return;
         }
+        if (!bound) (void)hipEventRecord(a, c->stream);
+    }
+    ccd::LaunchEvents ev_ = {};
+    const ccd::LaunchEvents* ev() {
+        if (!a) return nullptr;
+        ev_ = {a, b};
+        return &ev_;
     }
     ~Timed() {
-        if (c->prof && a) {
-            hipEvent_t b = take_event(c);
-            if (b) {
-                (void)hipEventRecord(b, c->stream);
-                c->recs.push_back({kind, a, b});
-            }
-        }
+        if (!a) return;
+        if (!bound) (void)hipEventRecord(b, c->stream);
+        c->recs.push_back({kind, a, b});
     }
 };
 
@@ -249,8 +266,8 @@ int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, co
     ccd::SweepArgs a{L.ptr, L.idx, L.val, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      L.units, L.partials, L.nunits, hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
-        Timed tm(c, kind);
-        HIP_TRY(c, ccd::launch_sweep(a, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream));
+        Timed tm(c, kind, true);
+        HIP_TRY(c, ccd::launch_sweep(a, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
     }
     if (sweep && L.nsplit) {
         Timed tm(c, CCD_K_FINALIZE);
@@ -403,8 +420,8 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
                      T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
-        Timed tm(c, kind);
-        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream));
+        Timed tm(c, kind, true);
+        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
@@ -534,6 +551,7 @@ int32_t ccd_destroy(ccd_ctx* c) {
     dfree(c->v16);
     dfree(c->rmse_bsum);
     dfree(c->rmse_dev);
+    drop_graph(c);
     for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -550,6 +568,7 @@ int32_t ccd_set_plan(ccd_ctx* c, const ccd_plan_params* pp) {
         return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8, 16 or 32");
     if (pp->blocks_per_cu > 2) return set_err(c, CCD_ERR_INVALID, "plan: blocks_per_cu must be 0, 1 or 2");
     c->plan = *pp;
+    ++c->gen;
     return CCD_OK;
 }
 
@@ -564,6 +583,8 @@ int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_
     if (const char* why = check_compressed(n, m, nnz, col_ptr, row_idx)) return set_err(c, CCD_ERR_INVALID, std::string("CSC: ") + why);
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    ++c->gen;
     free_layout(c->csr);
     free_layout(c->csc);
     c->uploaded = false;
@@ -641,6 +662,7 @@ int32_t ccd_set_factors(ccd_ctx* c, uint32_t k, const float* W, const float* H) 
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (k != c->k) {
+        drop_graph(c);
         dfree(c->W);
         dfree(c->H);
         c->W = c->H = nullptr;
@@ -727,6 +749,7 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     }
     c->iter_rmse.assign(T, NAN);
     c->prof = p->profile != 0;
+    c->prof_kinds = (uint32_t)p->profile_kinds;
     c->recs.clear();
     c->pool_used = 0;
     for (int q = 0; q < CCD_K_COUNT; ++q) {
@@ -789,7 +812,9 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     uint32_t prev = 0;
     int rc;
     size_t rec_mark = 0;
-    for (int oiter = 1; oiter <= T; ++oiter) {
+    // One outer iteration's ranks.  build_last_pairs = false: the commit of
+    // the run's last rank skips the pairs of a next rank that never comes.
+    auto outer = [&](int oiter, bool build_last_pairs) -> int {
         for (uint32_t t = 0; t < k; ++t) {
             const char* St = XS + (size_t)t * dS * es;
             const char* Ft = XF + (size_t)t * dF * es;
@@ -798,18 +823,18 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
             const bool add = oiter > 1 || c->continued;
             const bool fused = pending || add;
             if (sweeps) {
-                rc = pass(c, F, St, vF, Sp, Fp, St, Ft, true, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
-                if (rc) return rc;
-                rc = pass(c, S, vF, vS, Fp, Sp, Ft, St, false, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
-                if (rc) return rc;
+                int rc1 = pass(c, F, St, vF, Sp, Fp, St, Ft, true, pending, add, true, lambda, fused ? CCD_K_FUSED_F : CCD_K_SWEEP_F);
+                if (rc1) return rc1;
+                rc1 = pass(c, S, vF, vS, Fp, Sp, Ft, St, false, pending, add, true, lambda, fused ? CCD_K_FUSED_S : CCD_K_SWEEP_S);
+                if (rc1) return rc1;
                 for (int it = 2; it <= Tin; ++it) {
-                    rc = pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, false, true, lambda, CCD_K_SWEEP_F);
-                    if (rc) return rc;
-                    rc = pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, false, true, lambda, CCD_K_SWEEP_S);
-                    if (rc) return rc;
+                    rc1 = pass(c, F, vS, vF, nullptr, nullptr, nullptr, nullptr, false, false, false, true, lambda, CCD_K_SWEEP_F);
+                    if (rc1) return rc1;
+                    rc1 = pass(c, S, vF, vS, nullptr, nullptr, nullptr, nullptr, false, false, false, true, lambda, CCD_K_SWEEP_S);
+                    if (rc1) return rc1;
                 }
                 // W, H <- u, v; pairs of the next rank (none after the last)
-                const bool last = oiter == T && t + 1 == k;
+                const bool last = !build_last_pairs && oiter == T && t + 1 == k;
                 const uint32_t tn = t + 1 == k ? 0 : t + 1;
                 char* XSt = XS + (size_t)t * dS * es;
                 char* XFt = XF + (size_t)t * dF * es;
@@ -818,13 +843,65 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
                 Timed tm(c, CCD_K_COPY);
                 HIP_TRY(c, ccd::launch_commit(ca, bf, c->stream));
             } else if (fused) {
-                rc = pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, false, pending, add, false, lambda, CCD_K_RESIDUAL);
-                if (rc) return rc;
-                rc = pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, false, pending, add, false, lambda, CCD_K_RESIDUAL);
-                if (rc) return rc;
+                int rc1 = pass(c, F, nullptr, nullptr, Sp, Fp, St, Ft, false, pending, add, false, lambda, CCD_K_RESIDUAL);
+                if (rc1) return rc1;
+                rc1 = pass(c, S, nullptr, nullptr, Fp, Sp, Ft, St, false, pending, add, false, lambda, CCD_K_RESIDUAL);
+                if (rc1) return rc1;
             }
             pending = true;
             prev = t;
+        }
+        return CCD_OK;
+    };
+    // From outer iteration 2 on every iteration launches the same kernels
+    // with the same arguments (rank 0 subtracts rank k-1 and adds itself
+    // back), so with use_graph it is captured once and replayed.
+    const bool graph = p->use_graph != 0 && k > 0 && (sweeps || Tin == 0);
+    std::vector<uint64_t> gkey;
+    if (graph) {
+        uint32_t lam_bits;
+        std::memcpy(&lam_bits, &lambda, 4);
+        gkey = {c->gen, k, (uint64_t)Tin, lam_bits, (uint64_t)p->variant, (uint64_t)p->precision, (uint64_t)p->order,
+                c->plan.sigma, c->plan.blocks_per_cu, (uint64_t)(uintptr_t)XF, (uint64_t)(uintptr_t)XS,
+                (uint64_t)(uintptr_t)vF, (uint64_t)(uintptr_t)vS, (uint64_t)(uintptr_t)F.pair,
+                (uint64_t)(uintptr_t)S.pair, (uint64_t)(uintptr_t)F.tl.tval, (uint64_t)(uintptr_t)S.tl.tval};
+    }
+    for (int oiter = 1; oiter <= T; ++oiter) {
+        if (graph && oiter >= 2) {
+            if (!c->graph_exec || c->graph_key != gkey) {
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
+                drop_graph(c);
+                const bool prof = c->prof;
+                c->prof = false;  // no event records inside the capture
+                HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                const bool pending0 = pending;
+                const uint32_t prev0 = prev;
+                rc = outer(oiter, true);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(c->stream, &g);
+                c->prof = prof;
+                pending = pending0;  // nothing ran yet
+                prev = prev0;
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                HIP_TRY(c, e);
+                const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) c->graph_exec = nullptr;
+                HIP_TRY(c, ei);
+                c->graph_key = gkey;
+            }
+            {
+                Timed tm(c, -1);
+                HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
+            }
+            pending = true;
+            prev = k - 1;
+        } else {
+            rc = outer(oiter, false);
+            if (rc) return rc;
         }
         if (do_rmse) {
             Timed tm(c, CCD_K_RMSE);
@@ -871,6 +948,7 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     c->loop_s = 0.0;
     if (c->prof) {
         for (const ProfRec& r : c->recs) {
+            if (r.kind < 0) continue;  // a replayed outer iteration
             float ms = 0.0f;
             HIP_TRY(c, hipEventElapsedTime(&ms, r.a, r.b));
             c->kind_ms[r.kind] += ms;

@@ -108,6 +108,7 @@ struct ccd_ctx {
     ccd_plan_params plan{4, 8192, 4096, 4096, 16, 0};
     // profiling
     bool prof = false;
+    uint32_t prof_kinds = 0;  // bitmask of timed kernel kinds (1 << ccd_kernel_kind)
     std::vector<ccdi::ProfRec> recs;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -115,6 +116,10 @@ struct ccd_ctx {
     uint64_t kind_n[CCD_K_COUNT] = {};
     double kind_bytes[CCD_K_COUNT] = {};
     double loop_s = 0.0;
+    // outer iteration captured for replay (use_graph), valid for graph_key
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    uint64_t gen = 0;  // bumped whenever buffers or the plan change
 };
 
 namespace ccdi {
@@ -135,6 +140,7 @@ hipError_t dalloc(T** p, size_t count) {
 }
 
 void dfree(void* p);
+void drop_graph(ccd_ctx* c);
 // Tile descriptors of layout L for tiles of 64 * sigma elements (cached).
 int build_tiled(ccd_ctx* c, DevLayout& L, int sigma);
 

@@ -135,9 +135,17 @@ struct FinalizeArgs {
 // Dynamic LDS of a persistent sweep: hot_x factor entries, then hot_p pairs.
 size_t staged_lds_bytes(int prec, uint32_t hot_x, uint32_t hot_p);
 
+// Optional timing of one launch: events bound to the kernel's own dispatch
+// (hipExtLaunchKernel), so timing adds no packets between kernels.  When no
+// kernel is launched both events are recorded on the stream instead.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+hipError_t record_unlaunched(const LaunchEvents* ev, hipStream_t s);
+
 // blocks: persistent grid size (a multiple of the CU count)
 hipError_t launch_sweep(const SweepArgs& a, int prec, bool idx16, bool sub, bool add, bool sweep, uint32_t blocks,
-                        hipStream_t s);
+                        hipStream_t s, const LaunchEvents* ev = nullptr);
 hipError_t launch_finalize(const FinalizeArgs& a, int prec, hipStream_t s);
 // Blocks of one persistent sweep kernel that fit on a CU by register use
 // (the caller sizes the staged LDS prefix to match).
@@ -256,7 +264,7 @@ struct SpanArgs {
 
 // sigma 8, 16 or 32 for FP32; 16 for the other precisions
 hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
-                        uint32_t blocks, hipStream_t s);
+                        uint32_t blocks, hipStream_t s, const LaunchEvents* ev = nullptr);
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s);
 int tiled_blocks_per_cu(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool x_is_cur);
 // original order <-> tiled order (to_tiled = true: src original, dst tiled)

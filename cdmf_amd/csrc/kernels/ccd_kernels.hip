@@ -348,18 +348,25 @@ size_t staged_lds_bytes(int prec, uint32_t hot_x, uint32_t hot_p) {
     return (((size_t)hot_x * es + 15) & ~(size_t)15) + (size_t)hot_p * 2 * es;
 }
 
+hipError_t record_unlaunched(const LaunchEvents* ev, hipStream_t s) {
+    if (!ev || !ev->start || !ev->stop) return hipSuccess;
+    const hipError_t e = hipEventRecord(ev->start, s);
+    return e != hipSuccess ? e : hipEventRecord(ev->stop, s);
+}
+
 hipError_t launch_sweep(const SweepArgs& a, int prec, bool idx16, bool sub, bool add, bool sweep, uint32_t blocks,
-                        hipStream_t s) {
-    if (!sub && !add && !sweep) return hipSuccess;
+                        hipStream_t s, const LaunchEvents* ev) {
+    if (!sub && !add && !sweep) return record_unlaunched(ev, s);
     const SweepFn fn = sweep_fn(prec, idx16, sub, add, sweep);
     if (!fn) return hipErrorInvalidValue;
     const uint32_t need = (a.nunits + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
-    if (need == 0) return hipSuccess;
+    if (need == 0) return record_unlaunched(ev, s);
     if (blocks > need) blocks = need;
     SweepArgs arg = a;
     void* args[] = {&arg};
-    return hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
-                           staged_lds_bytes(prec, a.hot_x, a.hot_p), s);
+    return hipExtLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
+                              staged_lds_bytes(prec, a.hot_x, a.hot_p), s, ev ? ev->start : nullptr,
+                              ev ? ev->stop : nullptr, 0);
 }
 
 int sweep_blocks_per_cu(int prec, bool idx16, bool sub, bool add, bool sweep) {

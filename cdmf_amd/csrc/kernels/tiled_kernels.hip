@@ -144,17 +144,18 @@ TiledFn tiled_fn(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep
 }  // namespace
 
 hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
-                        uint32_t blocks, hipStream_t s) {
-    if (!sub && !add && !sweep) return hipSuccess;
+                        uint32_t blocks, hipStream_t s, const LaunchEvents* ev) {
+    if (!sub && !add && !sweep) return record_unlaunched(ev, s);
     const TiledFn fn = tiled_fn(sigma, prec, idx16, sub, add, sweep, a.x_is_cur != 0);
     if (!fn) return hipErrorInvalidValue;
     const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
-    if (need == 0) return hipSuccess;
+    if (need == 0) return record_unlaunched(ev, s);
     if (blocks > need) blocks = need;
     TiledArgs arg = a;
     void* args[] = {&arg};
-    return hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
-                           staged_lds_bytes(prec, a.hot_x, a.hot_p), s);
+    return hipExtLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
+                              staged_lds_bytes(prec, a.hot_x, a.hot_p), s, ev ? ev->start : nullptr,
+                              ev ? ev->stop : nullptr, 0);
 }
 
 int tiled_blocks_per_cu(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool xcur) {

@@ -34,7 +34,7 @@ class CParams(C.Structure):
         ("k", C.c_uint32), ("lambda_", C.c_float), ("maxiter", C.c_int32), ("maxinneriter", C.c_int32),
         ("variant", C.c_int32), ("precision", C.c_int32), ("order", C.c_int32), ("zero_first", C.c_int32),
         ("rmse_every", C.c_int32), ("profile", C.c_int32), ("use_graph", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("profile_kinds", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -225,12 +225,15 @@ class Solver:
     def run(self, k: int, lam: float, outer: int, inner: int, variant: int = VARIANT_NATIVE,
             precision: int = PREC_FP32, order: int = ORDER_H_FIRST, zero_first: bool = True,
             rmse_every: bool = True, profile: bool = False, use_graph: bool = False,
-            callback=None) -> RunResult:
+            callback=None, profile_kinds=()) -> RunResult:
+        """profile_kinds: names from KERNEL_KINDS to time when profile is set
+        (empty: every kind)."""
         p = CParams()
         self._lib.ccd_params_default(C.byref(p))
         p.k, p.lambda_, p.maxiter, p.maxinneriter = k, lam, outer, inner
         p.variant, p.precision, p.order = variant, precision, order
         p.zero_first, p.rmse_every, p.profile, p.use_graph = int(zero_first), int(rmse_every), int(profile), int(use_graph)
+        p.profile_kinds = sum(1 << KERNEL_KINDS.index(kk) for kk in set(profile_kinds))
         res = RunResult()
 
         def _cb(_user, oiter, rank_t, upd_t, rmse, rmse_t):

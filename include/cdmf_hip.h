@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CCD_ABI_VERSION 2
+#define CCD_ABI_VERSION 3
 
 /* Status codes: 0 = success, negative = error (message: ccd_last_error). */
 #define CCD_OK 0
@@ -78,9 +78,13 @@ typedef struct ccd_params {
     int32_t zero_first;   /* 1: zero the first-swept factor at entry (the reference zeroes H,
                              cdmf_ref.cpp:57-61); 0: keep it (the product W H^T must then be 0) */
     int32_t rmse_every;   /* 1: test RMSE after every outer iteration (needs ccd_upload_test) */
-    int32_t profile;      /* 1: time every kernel with HIP events (ccd_kernel_stats) */
+    int32_t profile;      /* 1: time kernels with HIP events (ccd_kernel_stats, ccd_loop_time) */
     int32_t use_graph;    /* 1: replay each outer iteration from a captured HIP graph */
-    int32_t reserved[5];  /* zero */
+    int32_t profile_kinds; /* with profile = 1: bitmask of (1 << ccd_kernel_kind) to time, 0 = every kind.
+                              Sweep kernels are timed by events bound to their own dispatch, which adds
+                              nothing between kernels; the other kinds by event records around the launch,
+                              which add a few microseconds between kernels each. */
+    int32_t reserved[4];  /* zero */
 } ccd_params;
 
 /* Fills the reference defaults of src/pmf.h:4-26 (k=10, lambda=0.1, 5 outer,

@@ -161,6 +161,63 @@ def test_deterministic(oracle_lib):
         assert np.array_equal(np.asarray(x), np.asarray(y))
 
 
+@pytest.mark.parametrize("variant", [1, 3])
+def test_graph_replay_bitwise(oracle_lib, variant):
+    """use_graph: outer iterations from 2 on are replayed from one captured
+    graph.  Same kernels, same arguments: bit-identical to eager launches,
+    for a fresh run, for a continued run that reuses the graph, and with
+    per-iteration callbacks."""
+    solver = _solver()
+    ds = _small()
+    k, lam, inner = 8, 0.05, 3
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    out = {}
+    for graph in (False, True):
+        _say("case graph=%s variant %d" % (graph, variant))
+        with solver.Solver(0) as s:
+            s.upload(ds)
+            s.set_factors(W0, H0)
+            iters = []
+            r1 = s.run(k, lam, 4, inner, variant=variant, use_graph=graph, profile=True, callback=iters.append)
+            a = (*s.factors(), *s.residual())
+            r2 = s.run(k, lam, 3, inner, variant=variant, use_graph=graph, zero_first=False)
+            b = (*s.factors(), *s.residual())
+            assert len(iters) == 4 and all(it.rank_time > 0 for it in iters)
+            out[graph] = (r1.rmse, a, r2.rmse, b)
+    e, g = out[False], out[True]
+    assert e[0] == g[0] and e[2] == g[2]
+    for x, y in zip(e[1] + e[3], g[1] + g[3]):
+        assert np.array_equal(x, y)
+    ref = oracle_lib.ccd(ds, W0, H0, k, lam, 4, inner)
+    _compare(*g[1][:2], g[0], *g[1][2:], *ref, "graph replay")
+
+
+def test_profile_kinds():
+    """profile_kinds times only the chosen kernel classes; the counts follow
+    the loop: the very first sweep of a run is plain, every other rank's
+    first sweep is fused."""
+    solver = _solver()
+    ds = _small()
+    k, outer, inner = 6, 2, 3
+    W0 = np.full((k, ds.m), 0.1, np.float32)
+    H0 = np.zeros((k, ds.n), np.float32)
+    with solver.Solver(0) as s:
+        s.upload(ds)
+        for kinds in (("sweep_f",), ()):
+            s.set_factors(W0, H0)
+            s.run(k, 0.05, outer, inner, variant=3, profile=True, profile_kinds=kinds)
+            st = s.kernel_stats()
+            _say("profile_kinds %s: %s" % (kinds, {kk: v["launches"] for kk, v in st.items()}))
+            assert st["sweep_f"]["launches"] == outer * k * (inner - 1) + 1
+            assert st["sweep_f"]["total_ms"] > 0
+            if kinds:
+                assert all(v["launches"] == 0 for kk, v in st.items() if kk != "sweep_f")
+            else:
+                assert st["fused_f"]["launches"] == outer * k - 1
+                assert st["sweep_s"]["launches"] == outer * k * (inner - 1) + 1
+                assert st["rmse"]["launches"] == outer
+
+
 def test_rmse_kernel_matches_oracle(oracle_lib):
     solver = _solver()
     ds = _small()
