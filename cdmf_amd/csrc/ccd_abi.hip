@@ -347,7 +347,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
     const int64_t rows = L.rows;
     std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows, span;
     std::vector<uint16_t> seg0(ntiles * 64);
-    std::vector<uint8_t> open(ntiles);
+    std::vector<uint8_t> open((ntiles + 3) & ~(uint64_t)3);  // whole words: the kernels read it a word at a time
     std::vector<uint32_t> emp(rows + 1, 0u);
     for (int64_t r = 0; r < rows; ++r) {
         const bool empty = ptr[r + 1] == ptr[r];

@@ -118,6 +118,7 @@ __device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, uint3
     const BatchLayout& L = *Lp;
     const TiledArgs a = layout_args(L, v);
     const int lane = threadIdx.x & (WAVE - 1);
+    w = __builtin_amdgcn_readfirstlane(w);  // wave-uniform: scalar loads of the tile descriptors
     if (L.idx16) {
         for (uint32_t t = w; t < L.ntiles; t += nw)
             tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR>(a, nullptr, nullptr, t, lane);

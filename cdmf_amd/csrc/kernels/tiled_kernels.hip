@@ -44,7 +44,9 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
     const uint32_t stride = gridDim.x * NW;
-    for (uint32_t tile = blockIdx.x * NW + (threadIdx.x >> 6); tile < a.ntiles; tile += stride)
+    // wave-uniform tile index: the per-tile descriptors become scalar loads
+    const uint32_t first = __builtin_amdgcn_readfirstlane(blockIdx.x * NW + (threadIdx.x >> 6));
+    for (uint32_t tile = first; tile < a.ntiles; tile += stride)
         tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane);
 }
 
