@@ -89,7 +89,7 @@ ccd::BatchLayout batch_layout(const ccdi::DevLayout& L) {
     d.nnz = L.nnz;
     d.ntiles = T.ntiles;
     d.nspan = T.nspan;
-    d.idx16 = L.idx16 ? 1u : 0u;
+    d.ib = (uint32_t)T.ib;
     d.rows = (uint32_t)L.rows;
     return d;
 }

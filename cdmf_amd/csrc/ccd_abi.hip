@@ -174,8 +174,10 @@ struct Timed {
 // streams (values read, and written back when the residual is updated), the
 // row pointer and the row list, one read of every distinct gathered vector
 // and per-row vector, and the output vector.
-double layout_bytes(const DevLayout& L, bool residual_write, int gathered, int per_row, bool writes_y) {
-    const double ib = L.idx16 ? 2.0 : 4.0;
+int tiled_index_bytes(const DevLayout& L) { return L.gdim <= 65536 ? 2 : L.gdim <= (1 << 24) ? 3 : 4; }
+
+double layout_bytes(const DevLayout& L, bool tiled, bool residual_write, int gathered, int per_row, bool writes_y) {
+    const double ib = tiled ? (double)tiled_index_bytes(L) : L.idx16 ? 2.0 : 4.0;
     double b = (double)L.nnz * (ib + 4.0 + (residual_write ? 4.0 : 0.0));
     b += 4.0 * (double)(L.rows + 1) + 4.0 * (double)L.rows;
     b += 4.0 * (double)L.gdim * gathered + 4.0 * (double)L.rows * per_row;
@@ -198,17 +200,18 @@ struct Launch {
     size_t lds_budget;
 };
 
-Launch launch_shape(ccd_ctx* c, bool tiled, int sigma, bool idx16, bool sub, bool add, bool sweep, bool xcur) {
-    const uint32_t key = (xcur ? 1u << 17 : 0u) | (tiled ? 1u << 16 : 0u) | (uint32_t)sigma << 8 |
-                         (uint32_t)c->prec << 4 | (idx16 ? 8u : 0u) | (sub ? 4u : 0u) | (add ? 2u : 0u) |
+// ib: bytes per index (2 or 4 for the binned sweeps, 2, 3 or 4 for the tiled ones)
+Launch launch_shape(ccd_ctx* c, bool tiled, int sigma, int ib, bool sub, bool add, bool sweep, bool xcur) {
+    const uint32_t key = (uint32_t)ib << 20 | (xcur ? 1u << 17 : 0u) | (tiled ? 1u << 16 : 0u) |
+                         (uint32_t)sigma << 8 | (uint32_t)c->prec << 4 | (sub ? 4u : 0u) | (add ? 2u : 0u) |
                          (sweep ? 1u : 0u);
     auto it = c->occupancy.find(key);
     int bpc;
     if (it != c->occupancy.end()) {
         bpc = it->second;
     } else {
-        bpc = tiled ? ccd::tiled_blocks_per_cu(sigma, c->prec, idx16, sub, add, sweep, xcur)
-                    : ccd::sweep_blocks_per_cu(c->prec, idx16, sub, add, sweep);
+        bpc = tiled ? ccd::tiled_blocks_per_cu(sigma, c->prec, ib, sub, add, sweep, xcur)
+                    : ccd::sweep_blocks_per_cu(c->prec, ib == 2, sub, add, sweep);
         c->occupancy[key] = bpc;
     }
     bpc = std::max(1, std::min(bpc, 2));
@@ -261,7 +264,7 @@ Hot hot_prefix(const ccd_ctx* c, DevLayout& L, size_t budget, bool fused, bool s
 int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
             const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
     const bool fused = sweep && (sub || add);
-    const Launch sh = launch_shape(c, false, 0, L.idx16, sub, add, sweep, false);
+    const Launch sh = launch_shape(c, false, 0, L.idx16 ? 2 : 4, sub, add, sweep, false);
     const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::SweepArgs a{L.ptr, L.idx, L.val, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      L.units, L.partials, L.nunits, hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
@@ -396,16 +399,13 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
     HIP_TRY(c, dalloc(&T.tval, padded));
     HIP_TRY(c, dalloc(&T.head, ntiles));
     HIP_TRY(c, dalloc(&T.tail, ntiles));
-    if (L.idx16) {
-        uint16_t* d = nullptr;
-        HIP_TRY(c, dalloc(&d, padded));
-        T.tidx = d;
-    } else {
-        uint32_t* d = nullptr;
-        HIP_TRY(c, dalloc(&d, padded));
+    T.ib = tiled_index_bytes(L);
+    {
+        uint8_t* d = nullptr;  // the 3-byte format: u16 plane, then u8 plane
+        HIP_TRY(c, dalloc(&d, padded * (size_t)T.ib));
         T.tidx = d;
     }
-    if (ntiles) HIP_TRY(c, ccd::launch_retile_idx(L.idx, T.tidx, L.idx16, nnz, T.ntiles, sigma, c->stream));
+    if (ntiles) HIP_TRY(c, ccd::launch_retile_idx(L.idx, L.idx16, T.tidx, T.ib, nnz, T.ntiles, sigma, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
@@ -414,14 +414,14 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
                   const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
     TiledLayout& T = L.tl;
     const bool fused = sweep && (sub || add);
-    const Launch sh = launch_shape(c, true, T.sigma, L.idx16, sub, add, sweep, fused && x_is_cur);
+    const Launch sh = launch_shape(c, true, T.sigma, T.ib, sub, add, sweep, fused && x_is_cur);
     const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
         Timed tm(c, kind, true);
-        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, L.idx16, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
+        HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, T.ib, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
@@ -756,11 +756,13 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         c->kind_ms[q] = 0.0;
         c->kind_n[q] = 0;
     }
-    c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, false, 1, 0, true);
-    c->kind_bytes[CCD_K_SWEEP_S] = layout_bytes(S, false, 1, 0, true);
-    c->kind_bytes[CCD_K_FUSED_F] = layout_bytes(F, true, 2, 2, true);
-    c->kind_bytes[CCD_K_FUSED_S] = layout_bytes(S, true, 3, 2, true);
-    c->kind_bytes[CCD_K_RESIDUAL] = 0.5 * (layout_bytes(F, true, 1, 1, false) + layout_bytes(S, true, 1, 1, false));
+    const bool tl = p->variant == CCD_VARIANT_TILED;
+    c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, tl, false, 1, 0, true);
+    c->kind_bytes[CCD_K_SWEEP_S] = layout_bytes(S, tl, false, 1, 0, true);
+    c->kind_bytes[CCD_K_FUSED_F] = layout_bytes(F, tl, true, 2, 2, true);
+    c->kind_bytes[CCD_K_FUSED_S] = layout_bytes(S, tl, true, 3, 2, true);
+    c->kind_bytes[CCD_K_RESIDUAL] =
+        0.5 * (layout_bytes(F, tl, true, 1, 1, false) + layout_bytes(S, tl, true, 1, 1, false));
     c->kind_bytes[CCD_K_FINALIZE] = 8.0 * (F.npieces + S.npieces) / 2.0;
     c->kind_bytes[CCD_K_RMSE] = 12.0 * (double)c->nnz_test + 4.0 * (double)k * (double)(c->m + c->n);
     c->kind_bytes[CCD_K_COPY] = 5.0 * (p->precision == CCD_PREC_BF16 ? 2.0 : 4.0) * (double)(c->m + c->n);

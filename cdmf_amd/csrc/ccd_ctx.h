@@ -16,6 +16,7 @@ namespace ccdi {
 // tiled run and written back to the layout's canonical order at its end.
 struct TiledLayout {
     int sigma = 0;
+    int ib = 0;  // bytes per index of tidx: 2, 3 or 4 (ccd_kernels.h)
     uint32_t ntiles = 0;
     void* tidx = nullptr;
     float* tval = nullptr;
@@ -141,6 +142,9 @@ hipError_t dalloc(T** p, size_t count) {
 
 void dfree(void* p);
 void drop_graph(ccd_ctx* c);
+// Bytes per index of the tiled copy of L: 2 up to 2^16 gathered entries, 3
+// up to 2^24, else 4.
+int tiled_index_bytes(const DevLayout& L);
 // Tile descriptors of layout L for tiles of 64 * sigma elements (cached).
 int build_tiled(ccd_ctx* c, DevLayout& L, int sigma);
 

@@ -119,12 +119,15 @@ __device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, uint3
     const TiledArgs a = layout_args(L, v);
     const int lane = threadIdx.x & (WAVE - 1);
     w = __builtin_amdgcn_readfirstlane(w);  // wave-uniform: scalar loads of the tile descriptors
-    if (L.idx16) {
+    if (L.ib == 2) {
         for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR>(a, nullptr, nullptr, t, lane);
+            tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, nullptr, nullptr, t, lane);
+    } else if (L.ib == 3) {
+        for (uint32_t t = w; t < L.ntiles; t += nw)
+            tiled::tile_body<Idx24, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, nullptr, nullptr, t, lane);
     } else {
         for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<uint32_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR>(a, nullptr, nullptr, t, lane);
+            tiled::tile_body<uint32_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, nullptr, nullptr, t, lane);
     }
 }
 

@@ -262,15 +262,23 @@ struct SpanArgs {
     float lambda;
 };
 
-// sigma 8, 16 or 32 for FP32; 16 for the other precisions
-hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
+// Index formats of the tiled copies, by bytes per index: 2 (u16), 3 (a u16
+// plane plus a u8 plane of bits 16..23, for gathered lengths up to 2^24) or
+// 4 (u32).  The 3-byte format keeps both planes in one allocation: the u16
+// plane of ntiles * 64 * sigma entries, then the u8 plane, where the high
+// bytes of lane l of tile t are the sigma bytes at t * 64 * sigma + l * sigma.
+struct Idx24 {};
+
+// sigma 8, 16 or 32 for FP32; 16 for the other precisions; ib: bytes per index
+hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, int ib, bool sub, bool add, bool sweep,
                         uint32_t blocks, hipStream_t s, const LaunchEvents* ev = nullptr);
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s);
-int tiled_blocks_per_cu(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool x_is_cur);
+int tiled_blocks_per_cu(int sigma, int prec, int ib, bool sub, bool add, bool sweep, bool x_is_cur);
 // original order <-> tiled order (to_tiled = true: src original, dst tiled)
 hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_t ntiles, int sigma, bool to_tiled,
                              hipStream_t s);
-hipError_t launch_retile_idx(const void* src, void* dst, bool idx16, uint64_t nnz, uint32_t ntiles, int sigma,
+// u16 or u32 indices in, the tiled copy in the format of ib out
+hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uint64_t nnz, uint32_t ntiles, int sigma,
                              hipStream_t s);
 hipError_t launch_fill(float* dst, float value, uint64_t count, hipStream_t s);
 hipError_t launch_fill_bytes16(uint16_t* dst, uint16_t value, uint64_t count, hipStream_t s);
@@ -303,7 +311,7 @@ struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
     uint64_t nnz;
     uint32_t ntiles;
     uint32_t nspan;
-    uint32_t idx16;
+    uint32_t ib;  // bytes per index of the tiled copy: 2, 3 or 4
     uint32_t rows;
 };
 

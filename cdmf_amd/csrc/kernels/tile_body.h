@@ -4,18 +4,24 @@
 #pragma once
 #include "ccd_kernels.h"
 
+#include <type_traits>
+
 namespace ccd {
 namespace tiled {
 
 constexpr int WAVE = 64;
 
-// Row of segment `seg` of a tile: consecutive rows from row0, or (SEGTAB:
-// the tile holds empty rows) from the tile's slice of the segment table.
-template <bool SEGTAB>
+// Row of segment `seg` of a tile: consecutive rows from row0, or, for a
+// tile that holds empty rows, from the tile's slice of the segment table.
+// SEG: SEG_RUN / SEG_TABLE when the caller knows which, SEG_ANY to decide
+// per tile from segoff.
+constexpr int SEG_RUN = 0, SEG_TABLE = 1, SEG_ANY = 2;
+template <int SEG>
 __device__ __forceinline__ uint32_t seg_row(uint32_t seg, uint32_t row0, uint32_t segoff,
                                             const uint32_t* __restrict__ segrows) {
-    if constexpr (SEGTAB) return segrows[segoff + seg];
-    else return row0 + seg;
+    if constexpr (SEG == SEG_TABLE) return segrows[segoff + seg];
+    else if constexpr (SEG == SEG_RUN) return row0 + seg;
+    else return segoff == 0xffffffffu ? row0 + seg : segrows[segoff + seg];
 }
 
 template <class TA>
@@ -31,47 +37,92 @@ __device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA 
     stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));
 }
 
-// The per-lane stream words of one tile: raw index groups (four indices a
-// load), value groups, the lane's row-start flags and first segment.
+// Index plane element: u16 (also the low plane of the 3-byte format) or
+// u32; either way a lane reads 16 bytes a load (8 or 4 indices).
 template <typename IdxT>
-struct IdxGroup;
-template <>
-struct IdxGroup<uint16_t> {
-    using type = vu2;
+struct IdxGroup {
+    using elem = uint16_t;
 };
 template <>
 struct IdxGroup<uint32_t> {
-    using type = vu4;
+    using elem = uint32_t;
 };
 
+// High bytes of the 3-byte format: the lane's SIGMA bytes, one or two 16-byte
+// loads (8 bytes for SIGMA = 8).
 template <typename IdxT, int SIGMA>
-struct Streams {
-    typename IdxGroup<IdxT>::type ci[SIGMA / 4];
-    vf4 rv[SIGMA / 4];
-    uint32_t flags, seg0;
+struct HiPlane {
+    __device__ __forceinline__ void load(const TiledArgs&, uint32_t, int) {}
+    __device__ __forceinline__ uint32_t at(int) const { return 0u; }
+};
+template <int SIGMA>
+struct HiPlane<Idx24, SIGMA> {
+    uint32_t w[SIGMA / 4];
+    __device__ __forceinline__ void load(const TiledArgs& a, uint32_t tile, int lane) {
+        constexpr uint32_t TS = WAVE * SIGMA;
+        const uint8_t* hi = reinterpret_cast<const uint8_t*>(static_cast<const uint16_t*>(a.tidx) +
+                                                             (size_t)a.ntiles * TS) +
+                            (size_t)tile * TS + (size_t)lane * SIGMA;
+        if constexpr (SIGMA == 8) {
+            const vu2 v = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(hi));
+            w[0] = v.x;
+            w[1] = v.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < SIGMA / 16; ++q) {
+                const vu4 v = __builtin_nontemporal_load(reinterpret_cast<const vu4*>(hi) + q);
+                w[4 * q + 0] = v.x;
+                w[4 * q + 1] = v.y;
+                w[4 * q + 2] = v.z;
+                w[4 * q + 3] = v.w;
+            }
+        }
+    }
+    // bits 16..23 of element i, in place
+    __device__ __forceinline__ uint32_t at(int i) const { return ((w[i >> 2] >> (8 * (i & 3))) & 0xffu) << 16; }
 };
 
+// The lane's streams of one tile, unpacked: SIGMA indices and values, its
+// row-start flags and first segment.  Indices are loaded first: the gathers
+// wait for them, not for the values.
 template <typename IdxT, int SIGMA>
-__device__ __forceinline__ void load_streams(const TiledArgs& a, const uint32_t tile, const int lane,
-                                             Streams<IdxT, SIGMA>& s) {
-    using IV = typename IdxGroup<IdxT>::type;
+__device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t tile, const int lane, uint32_t (&c)[SIGMA],
+                                          float (&r)[SIGMA], uint32_t& flags, uint32_t& seg0) {
+    using IE = typename IdxGroup<IdxT>::elem;
+    constexpr int G = 16 / (int)sizeof(IE);  // indices per load
     constexpr uint32_t TS = WAVE * SIGMA;
-    const IdxT* __restrict__ tidx = static_cast<const IdxT*>(a.tidx) + (size_t)tile * TS;
+    const IE* __restrict__ tidx = static_cast<const IE*>(a.tidx) + (size_t)tile * TS;
     const float* __restrict__ tval = a.tval + (size_t)tile * TS;
-    s.flags = a.lane_flags[(size_t)tile * WAVE + lane];
-    s.seg0 = a.lane_seg0[(size_t)tile * WAVE + lane];
-    // indices first: the gathers wait for them, not for the values
+    flags = a.lane_flags[(size_t)tile * WAVE + lane];
+    seg0 = a.lane_seg0[(size_t)tile * WAVE + lane];
+    vu4 ci[SIGMA / G];
 #pragma unroll
-    for (int q = 0; q < SIGMA / 4; ++q)
-        s.ci[q] = __builtin_nontemporal_load(reinterpret_cast<const IV*>(tidx + q * 256 + 4 * lane));
+    for (int q = 0; q < SIGMA / G; ++q)
+        ci[q] = __builtin_nontemporal_load(reinterpret_cast<const vu4*>(tidx + q * G * WAVE + G * lane));
+    HiPlane<IdxT, SIGMA> hi;
+    hi.load(a, tile, lane);
 #pragma unroll
-    for (int q = 0; q < SIGMA / 4; ++q)
-        s.rv[q] = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(tval + q * 256 + 4 * lane));
+    for (int q = 0; q < SIGMA / 4; ++q) {
+        const vf4 rv = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(tval + q * 256 + 4 * lane));
+        r[4 * q + 0] = rv.x;
+        r[4 * q + 1] = rv.y;
+        r[4 * q + 2] = rv.z;
+        r[4 * q + 3] = rv.w;
+    }
+#pragma unroll
+    for (int q = 0; q < SIGMA / G; ++q) {
+        const uint32_t wd[4] = {ci[q].x, ci[q].y, ci[q].z, ci[q].w};
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if constexpr (G == 8) c[8 * q + j] = ((wd[j >> 1] >> (16 * (j & 1))) & 0xffffu) | hi.at(8 * q + j);
+            else c[4 * q + j] = wd[j];
+        }
+    }
 }
 
 // Residual update, per-row sums and row finishes of one tile (the part of
 // tile_body_s after the gathers).
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool SEGTAB>
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, int SEG>
 __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t tile, const int lane, const uint32_t flags,
                                           const uint32_t seg0, const uint32_t row0, const uint32_t segoff,
                                           const uint32_t open, const uint32_t (&c)[SIGMA], float (&r)[SIGMA], const float (&xsv)[SIGMA],
@@ -80,7 +131,7 @@ __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t til
     float* __restrict__ tval = a.tval + (size_t)tile * TS;
     const uint64_t lbase = (uint64_t)tile * TS + (uint64_t)lane * SIGMA;
     uint32_t seg = seg0;
-    uint32_t row = seg_row<SEGTAB>(seg, row0, segoff, a.segrows);
+    uint32_t row = seg_row<SEG>(seg, row0, segoff, a.segrows);
     float ysr = SUB ? ldf<TF>(a.ys, row) : 0.0f;
     float yar = ADD ? ldf<TF>(a.ya, row) : 0.0f;
     TA g = 0, h = 0, gA = 0, hA = 0;
@@ -105,7 +156,7 @@ __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t til
             g = 0;
             h = 0;
             ++seg;
-            row = seg_row<SEGTAB>(seg, row0, segoff, a.segrows);
+            row = seg_row<SEG>(seg, row0, segoff, a.segrows);
             if (SUB) ysr = ldf<TF>(a.ys, row);
             if (ADD) yar = ldf<TF>(a.ya, row);
         }
@@ -169,7 +220,7 @@ __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t til
             const TA th = bit0 ? ch : ch + hA;
             const int end = lane * SIGMA + (bit0 ? 0 : first);
             if (seg_in == 0 && (open & 1u)) static_cast<Pair<TA>*>(a.head)[tile] = Pair<TA>{tg, th};
-            else finish_row<TF, TA>(a, seg_row<SEGTAB>(seg_in, row0, segoff, a.segrows), tg, th, (uint32_t)(end - ls_in));
+            else finish_row<TF, TA>(a, seg_row<SEG>(seg_in, row0, segoff, a.segrows), tg, th, (uint32_t)(end - ls_in));
         }
     }
     if (lane == WAVE - 1) {
@@ -185,42 +236,24 @@ __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t til
 
 // XCUR (fused passes only): the sweep's x is the pair's current value.
 // `tile` is wave-uniform, so the per-tile descriptors are scalar loads.
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
-__device__ __forceinline__ void tile_body_s(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp,
-                                            const uint32_t tile, const int lane, const Streams<IdxT, SIGMA>& cur) {
+// SPLIT: separate code for tiles with and without empty rows (faster; the
+// batched kernel, which inlines every pass mode, keeps one copy to stay
+// within its registers).
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR,
+          bool SPLIT = true>
+__device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
+                                          const int lane) {
     constexpr bool PAIR = SWEEP && (SUB || ADD);
-    const uint32_t flags = cur.flags;
-    const uint32_t seg0 = cur.seg0;
+    uint32_t c[SIGMA];
+    float r[SIGMA];
+    uint32_t flags, seg0;
+    load_tile<IdxT, SIGMA>(a, tile, lane, c, r, flags, seg0);
     const uint32_t row0 = a.tile_row0[tile];
     const uint32_t segoff = a.tile_segoff[tile];
     // the byte of this tile, read as its 32-bit word (the array is padded to
     // whole words) so that a wave-uniform tile gives a scalar load
     const uint32_t open =
         (reinterpret_cast<const uint32_t*>(a.tile_open)[tile >> 2] >> (8u * (tile & 3u))) & 0xffu;
-
-    uint32_t c[SIGMA];
-    float r[SIGMA];
-#pragma unroll
-    for (int q = 0; q < SIGMA / 4; ++q) {
-        if constexpr (sizeof(IdxT) == 2) {
-            const vu2 cv = cur.ci[q];
-            c[4 * q + 0] = cv.x & 0xffffu;
-            c[4 * q + 1] = cv.x >> 16;
-            c[4 * q + 2] = cv.y & 0xffffu;
-            c[4 * q + 3] = cv.y >> 16;
-        } else {
-            const vu4 cv = cur.ci[q];
-            c[4 * q + 0] = cv.x;
-            c[4 * q + 1] = cv.y;
-            c[4 * q + 2] = cv.z;
-            c[4 * q + 3] = cv.w;
-        }
-        const vf4 rv = cur.rv[q];
-        r[4 * q + 0] = rv.x;
-        r[4 * q + 1] = rv.y;
-        r[4 * q + 2] = rv.z;
-        r[4 * q + 3] = rv.w;
-    }
 
     // Every gather of the lane is issued before the tile's first store, so
     // all of them are in flight together rather than one per element.
@@ -242,21 +275,15 @@ __device__ __forceinline__ void tile_body_s(const TiledArgs& a, const TF* lds, c
     // of the rest, chosen by a wave-uniform branch, so the common one has no
     // segment-table loads (a load on only one of two merged paths makes the
     // compiler wait for every outstanding load at each later use).
-    if (segoff == 0xffffffffu)
-        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, false>(a, tile, lane, flags, seg0, row0, segoff, open, c, r, xsv,
-                                                             xav, xv);
+    if constexpr (!SPLIT)
+        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, SEG_ANY>(a, tile, lane, flags, seg0, row0, segoff, open, c, r,
+                                                               xsv, xav, xv);
+    else if (segoff == 0xffffffffu)
+        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, SEG_RUN>(a, tile, lane, flags, seg0, row0, segoff, open, c, r,
+                                                               xsv, xav, xv);
     else
-        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, true>(a, tile, lane, flags, seg0, row0, segoff, open, c, r, xsv,
-                                                            xav, xv);
-}
-
-// One tile: its streams, then the body.
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
-__device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
-                                          const int lane) {
-    Streams<IdxT, SIGMA> s;
-    load_streams<IdxT, SIGMA>(a, tile, lane, s);
-    tile_body_s<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane, s);
+        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, SEG_TABLE>(a, tile, lane, flags, seg0, row0, segoff, open, c,
+                                                                 r, xsv, xav, xv);
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], by one

@@ -11,9 +11,10 @@
 // Persistent kernel: each 1024-thread block stages the hot prefixes of the
 // gathered vectors in LDS, then its waves take tiles in a grid-stride loop.
 // One wave per tile of 64*SIGMA elements.  Lane l owns elements
-// l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of four,
-// element i of lane l at (i/4)*256 + 4*l + i%4, so one 16-byte load per lane
-// moves 1 KiB of consecutive values (512 B of 16-bit indices) per wave.  The
+// l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of
+// G = 16 / (bytes per entry), element i of lane l at (i/G)*64G + G*l + i%G,
+// so every 16-byte load of a lane moves 1 KiB of consecutive values or
+// indices per wave (G = 4 for values and 32-bit indices, 8 for 16-bit ones).  The
 // streams are read and written with non-temporal hints so that they do not
 // evict the gathered factor vector from L2.  Each lane updates its elements
 // and keeps a running (g, h) for the row it is in.  A row that starts and ends inside a
@@ -79,8 +80,25 @@ __global__ __launch_bounds__(BLOCK) void retile_idx_kernel(const IdxT* __restric
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
         const uint64_t tile = e / ts, rem = e % ts;
         const uint64_t i = rem % sigma, l = rem / sigma;
-        const uint64_t pos = tile * ts + (i >> 2) * (4 * WAVE) + 4 * l + (i & 3);
+        constexpr uint64_t G = 16 / sizeof(IdxT);  // indices per 16-byte lane load
+        const uint64_t pos = tile * ts + (i / G) * (G * WAVE) + G * l + (i % G);
         dst[pos] = e < nnz ? src[e] : (IdxT)0;
+    }
+}
+
+// 32-bit indices (each < 2^24) to the 3-byte format: the low 16 bits at the
+// position of the u16 layout, bits 16..23 in the lane-major u8 plane.
+__global__ __launch_bounds__(BLOCK) void retile_idx24_kernel(const uint32_t* __restrict__ src,
+                                                             uint16_t* __restrict__ lo, uint8_t* __restrict__ hi,
+                                                             uint64_t nnz, uint64_t padded, uint32_t sigma) {
+    const uint64_t ts = (uint64_t)WAVE * sigma;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
+        const uint64_t tile = e / ts, rem = e % ts;
+        const uint64_t i = rem % sigma, l = rem / sigma;
+        const uint32_t v = e < nnz ? src[e] : 0u;
+        lo[tile * ts + (i >> 3) * (8 * WAVE) + 8 * l + (i & 7)] = (uint16_t)(v & 0xffffu);
+        hi[e] = (uint8_t)(v >> 16);  // e = tile * ts + l * sigma + i
     }
 }
 
@@ -137,18 +155,23 @@ TiledFn tiled_fn_i(int sigma, int prec, int code) {
     return nullptr;
 }
 
-TiledFn tiled_fn(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool xcur) {
+TiledFn tiled_fn(int sigma, int prec, int ib, bool sub, bool add, bool sweep, bool xcur) {
     const bool fused = sweep && (sub || add);
     const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0) | (fused && xcur ? 8 : 0);
-    return idx16 ? tiled_fn_i<uint16_t>(sigma, prec, code) : tiled_fn_i<uint32_t>(sigma, prec, code);
+    switch (ib) {
+        case 2: return tiled_fn_i<uint16_t>(sigma, prec, code);
+        case 3: return tiled_fn_i<Idx24>(sigma, prec, code);
+        case 4: return tiled_fn_i<uint32_t>(sigma, prec, code);
+        default: return nullptr;
+    }
 }
 
 }  // namespace
 
-hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, bool sub, bool add, bool sweep,
+hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, int ib, bool sub, bool add, bool sweep,
                         uint32_t blocks, hipStream_t s, const LaunchEvents* ev) {
     if (!sub && !add && !sweep) return record_unlaunched(ev, s);
-    const TiledFn fn = tiled_fn(sigma, prec, idx16, sub, add, sweep, a.x_is_cur != 0);
+    const TiledFn fn = tiled_fn(sigma, prec, ib, sub, add, sweep, a.x_is_cur != 0);
     if (!fn) return hipErrorInvalidValue;
     const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
     if (need == 0) return record_unlaunched(ev, s);
@@ -160,8 +183,8 @@ hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, bool idx16, boo
                               ev ? ev->stop : nullptr, 0);
 }
 
-int tiled_blocks_per_cu(int sigma, int prec, bool idx16, bool sub, bool add, bool sweep, bool xcur) {
-    const TiledFn fn = tiled_fn(sigma, prec, idx16, sub, add, sweep, xcur);
+int tiled_blocks_per_cu(int sigma, int prec, int ib, bool sub, bool add, bool sweep, bool xcur) {
+    const TiledFn fn = tiled_fn(sigma, prec, ib, sub, add, sweep, xcur);
     int nb = 0;
     if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), SWEEP_BLOCK, 0) !=
                    hipSuccess)
@@ -189,15 +212,24 @@ hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_retile_idx(const void* src, void* dst, bool idx16, uint64_t nnz, uint32_t ntiles, int sigma,
+hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uint64_t nnz, uint32_t ntiles, int sigma,
                              hipStream_t s) {
     const uint64_t padded = (uint64_t)ntiles * WAVE * (uint64_t)sigma;
-    if (idx16)
-        hipLaunchKernelGGL(retile_idx_kernel<uint16_t>, dim3(grid_for(padded)), dim3(BLOCK), 0, s,
-                           static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), nnz, padded, (uint32_t)sigma);
-    else
-        hipLaunchKernelGGL(retile_idx_kernel<uint32_t>, dim3(grid_for(padded)), dim3(BLOCK), 0, s,
-                           static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), nnz, padded, (uint32_t)sigma);
+    const dim3 grid(grid_for(padded));
+    if (src16 != (ib == 2)) return hipErrorInvalidValue;
+    if (ib == 2) {
+        hipLaunchKernelGGL(retile_idx_kernel<uint16_t>, grid, dim3(BLOCK), 0, s, static_cast<const uint16_t*>(src),
+                           static_cast<uint16_t*>(dst), nnz, padded, (uint32_t)sigma);
+    } else if (ib == 3) {
+        uint16_t* lo = static_cast<uint16_t*>(dst);
+        hipLaunchKernelGGL(retile_idx24_kernel, grid, dim3(BLOCK), 0, s, static_cast<const uint32_t*>(src), lo,
+                           reinterpret_cast<uint8_t*>(lo + padded), nnz, padded, (uint32_t)sigma);
+    } else if (ib == 4) {
+        hipLaunchKernelGGL(retile_idx_kernel<uint32_t>, grid, dim3(BLOCK), 0, s, static_cast<const uint32_t*>(src),
+                           static_cast<uint32_t*>(dst), nnz, padded, (uint32_t)sigma);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
