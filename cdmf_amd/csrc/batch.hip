@@ -84,7 +84,7 @@ ccd::BatchLayout batch_layout(const ccdi::DevLayout& L) {
     d.tile_open = T.open;
     d.head = T.head;
     d.tail = T.tail;
-    d.span_rows = T.span_rows;
+    d.spans = T.spans;
     d.pair = L.pair;
     d.nnz = L.nnz;
     d.ntiles = T.ntiles;

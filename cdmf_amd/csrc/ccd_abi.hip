@@ -55,7 +55,7 @@ void free_tiled(TiledLayout& T) {
     dfree(T.open);
     dfree(T.head);
     dfree(T.tail);
-    dfree(T.span_rows);
+    dfree(T.spans);
     T = TiledLayout{};
 }
 
@@ -348,7 +348,8 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
     if (ntiles >= 0xffffffffull / 64) return set_err(c, CCD_ERR_INVALID, "too many tiles");
     const uint32_t* ptr = L.hptr.data();
     const int64_t rows = L.rows;
-    std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows, span;
+    std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows;
+    std::vector<ccd::SpanRow> span;
     std::vector<uint16_t> seg0(ntiles * 64);
     std::vector<uint8_t> open((ntiles + 3) & ~(uint64_t)3);  // whole words: the kernels read it a word at a time
     std::vector<uint32_t> emp(rows + 1, 0u);
@@ -358,7 +359,8 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
         if (empty) continue;
         const uint64_t e = ptr[r], rem = e % TS;
         flags[(e / TS) * 64 + rem / sigma] |= 1u << (rem % sigma);
-        if (e / TS != (ptr[r + 1] - 1ull) / TS) span.push_back((uint32_t)r);
+        const uint64_t t0 = e / TS, t1 = (ptr[r + 1] - 1ull) / TS;
+        if (t0 != t1) span.push_back({(uint32_t)r, (uint32_t)t0, (uint32_t)t1, ptr[r + 1] - ptr[r]});
     }
     int64_t r = 0;
     for (uint64_t t = 0; t < ntiles; ++t) {
@@ -393,7 +395,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
     if ((rc = upload_vec(c, &T.flags, flags)) || (rc = upload_vec(c, &T.seg0, seg0)) ||
         (rc = upload_vec(c, &T.row0, row0)) || (rc = upload_vec(c, &T.segoff, segoff)) ||
         (rc = upload_vec(c, &T.segrows, segrows)) || (rc = upload_vec(c, &T.open, open)) ||
-        (rc = upload_vec(c, &T.span_rows, span)))
+        (rc = upload_vec(c, &T.spans, span)))
         return rc;
     const uint64_t padded = ntiles * TS;
     HIP_TRY(c, dalloc(&T.tval, padded));
@@ -425,7 +427,7 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
-        ccd::SpanArgs sa{T.span_rows, L.ptr, T.head, T.tail, y, T.nspan, 64u * (uint32_t)T.sigma, lambda};
+        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, lambda};
         HIP_TRY(c, ccd::launch_span(sa, c->prec, c->stream));
     }
     return CCD_OK;

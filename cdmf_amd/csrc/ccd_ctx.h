@@ -28,7 +28,7 @@ struct TiledLayout {
     uint8_t* open = nullptr;
     double2* head = nullptr;  // (g, h) pairs; double-sized so every precision fits
     double2* tail = nullptr;
-    uint32_t* span_rows = nullptr;
+    ccd::SpanRow* spans = nullptr;  // rows spanning tiles
     uint32_t nspan = 0;
 };
 

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t w = rank * WPB + threadIdx.x / WAVE, nw = P.nblocks * WPB;
     const uint32_t tid = rank * BATCH_BLOCK + threadIdx.x, nth = P.nblocks * BATCH_BLOCK;
-    const uint32_t dF = P.F.rows, dS = P.S.rows, TE = WAVE * SIGMA;
+    const uint32_t dF = P.F.rows, dS = P.S.rows;
     const float lambda = P.lambda;
 
     // one sweep (or fused) pass: tiles, barrier, rows spanning tiles, barrier
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
         tiles<SIGMA>(&L, pv, sub, add, true, w, nw);
         if (!team_sync(tm)) return false;
         if (L.nspan) {
-            const SpanArgs sa{L.span_rows, L.ptr, L.head, L.tail, pv.y, L.nspan, TE, lambda};
+            const SpanArgs sa{L.spans, L.head, L.tail, pv.y, L.nspan, lambda};
             for (uint32_t i = w; i < L.nspan; i += nw) tiled::span_row<float, float>(sa, i, lane);
             if (!team_sync(tm)) return false;
         }

@@ -251,14 +251,18 @@ struct TiledArgs {
     float lambda;
 };
 
+// One row spanning tiles t0..t1 (t1 > t0) of a tiled layout: its row, first
+// and last tile and length.
+struct SpanRow {
+    uint32_t row, t0, t1, len;
+};
+
 struct SpanArgs {
-    const uint32_t* span_rows;
-    const uint32_t* ptr;
+    const SpanRow* spans;
     const void* head;
     const void* tail;
     void* y;
     uint32_t nspan;
-    uint32_t tile_elems;
     float lambda;
 };
 
@@ -306,7 +310,7 @@ struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
     const uint8_t* tile_open;
     void* head;
     void* tail;
-    const uint32_t* span_rows;
+    const SpanRow* spans;
     void* pair;  // FPair<float>[gathered length]
     uint64_t nnz;
     uint32_t ntiles;

@@ -287,15 +287,17 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], by one
-// wave, lane-strided partial sums reduced in a fixed tree.
+// wave, lane-strided partial sums reduced in a fixed tree.  The span table
+// holds the tile range and length, so the partials are the only dependent
+// loads.
 template <typename TF, typename TA>
 __device__ __forceinline__ void span_row(const SpanArgs& a, uint32_t i, int lane) {
-    const uint32_t row = a.span_rows[i];
-    const uint32_t s = a.ptr[row], e = a.ptr[row + 1];
-    const uint32_t t0 = s / a.tile_elems, t1 = (e - 1) / a.tile_elems;
+    const SpanRow sp = a.spans[i];
     const Pair<TA>* __restrict__ head = static_cast<const Pair<TA>*>(a.head);
+    Pair<TA> tl{0, 0};
+    if (lane == 0) tl = static_cast<const Pair<TA>*>(a.tail)[sp.t0];
     TA g = 0, h = 0;
-    for (uint32_t t = t0 + 1 + lane; t <= t1; t += WAVE) {
+    for (uint32_t t = sp.t0 + 1 + lane; t <= sp.t1; t += WAVE) {
         const Pair<TA> p = head[t];
         g += p.g;
         h += p.h;
@@ -306,10 +308,9 @@ __device__ __forceinline__ void span_row(const SpanArgs& a, uint32_t i, int lane
         h += __shfl_xor(h, off);
     }
     if (lane == 0) {
-        const Pair<TA> p = static_cast<const Pair<TA>*>(a.tail)[t0];
-        g = p.g + g;
-        h = p.h + h;
-        stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)(e - s) + h)));
+        g = tl.g + g;
+        h = tl.h + h;
+        stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
     }
 }
 
