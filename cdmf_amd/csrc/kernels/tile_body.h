@@ -120,20 +120,52 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
     }
 }
 
-// Residual update, per-row sums and row finishes of one tile (the part of
-// tile_body_s after the gathers).
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, int SEG>
-__device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t tile, const int lane, const uint32_t flags,
-                                          const uint32_t seg0, const uint32_t row0, const uint32_t segoff,
-                                          const uint32_t open, const uint32_t (&c)[SIGMA], float (&r)[SIGMA], const float (&xsv)[SIGMA],
-                                          const float (&xav)[SIGMA], const float (&xv)[SIGMA]) {
+// One tile after its streams are loaded: the lane's per-row vectors and
+// gathers, the residual update, per-row sums and row finishes.  SEG: how the
+// tile's rows are found (seg_row).  XCUR (fused passes only): the sweep's x
+// is the pair's current value.
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR, int SEG>
+__device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
+                                          const int lane, const uint32_t flags, const uint32_t seg0,
+                                          const uint32_t row0, const uint32_t segoff, const uint32_t open,
+                                          const uint32_t (&c)[SIGMA], float (&r)[SIGMA]) {
+    constexpr bool PAIR = SWEEP && (SUB || ADD);
     constexpr uint32_t TS = WAVE * SIGMA;
     float* __restrict__ tval = a.tval + (size_t)tile * TS;
     const uint64_t lbase = (uint64_t)tile * TS + (uint64_t)lane * SIGMA;
     uint32_t seg = seg0;
     uint32_t row = seg_row<SEG>(seg, row0, segoff, a.segrows);
+    // Per-row vectors of the lane's first two rows (a lane of SIGMA elements
+    // rarely holds more), loaded before the gathers so that their latency
+    // overlaps; a third row's are loaded where it starts.
     float ysr = SUB ? ldf<TF>(a.ys, row) : 0.0f;
     float yar = ADD ? ldf<TF>(a.ya, row) : 0.0f;
+    float ysn = 0.0f, yan = 0.0f;
+    if constexpr (SUB || ADD) {
+        if (flags >> 1) {
+            const uint32_t rn = seg_row<SEG>(seg + 1, row0, segoff, a.segrows);
+            if (SUB) ysn = ldf<TF>(a.ys, rn);
+            if (ADD) yan = ldf<TF>(a.ya, rn);
+        }
+    }
+
+    // Every gather of the lane is issued before the tile's first store, so
+    // all of them are in flight together rather than one per element.
+    float xsv[SIGMA], xav[SIGMA], xv[SIGMA];
+#pragma unroll
+    for (int i = 0; i < SIGMA; ++i) {
+        if constexpr (PAIR) {
+            const FPair<TF> pr = gather_pair<TF>(ldsp, a.hot_p, a.xp, c[i]);
+            xsv[i] = SUB ? tof<TF>(pr.prev) : 0.0f;
+            xav[i] = tof<TF>(pr.cur);
+            xv[i] = XCUR ? xav[i] : gather<TF>(lds, a.hot_x, a.x, c[i]);
+        } else {
+            xsv[i] = SUB ? ldf<TF>(a.xs, c[i]) : 0.0f;
+            xav[i] = ADD ? ldf<TF>(a.xa, c[i]) : 0.0f;
+            xv[i] = SWEEP ? gather<TF>(lds, a.hot_x, a.x, c[i]) : 0.0f;
+        }
+    }
+
     TA g = 0, h = 0, gA = 0, hA = 0;
     bool has_start = (flags & 1u) != 0;
     // st: lane index where the current row started (valid once has_start);
@@ -155,10 +187,14 @@ __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t til
             has_start = true;
             g = 0;
             h = 0;
-            ++seg;
-            row = seg_row<SEG>(seg, row0, segoff, a.segrows);
-            if (SUB) ysr = ldf<TF>(a.ys, row);
-            if (ADD) yar = ldf<TF>(a.ya, row);
+            row = seg_row<SEG>(++seg, row0, segoff, a.segrows);
+            if (seg == seg0 + 1) {
+                ysr = ysn;
+                yar = yan;
+            } else {
+                if (SUB) ysr = ldf<TF>(a.ys, row);
+                if (ADD) yar = ldf<TF>(a.ya, row);
+            }
         }
         const bool valid = lbase + (uint64_t)i < a.nnz;
         float v = r[i];
@@ -234,16 +270,14 @@ __device__ __forceinline__ void tile_rows(const TiledArgs& a, const uint32_t til
     }
 }
 
-// XCUR (fused passes only): the sweep's x is the pair's current value.
-// `tile` is wave-uniform, so the per-tile descriptors are scalar loads.
-// SPLIT: separate code for tiles with and without empty rows (faster; the
+// One tile: streams, then tile_core.  `tile` is wave-uniform, so the
+// per-tile descriptors are scalar loads.  SPLIT: separate code for tiles with and without empty rows (faster; the
 // batched kernel, which inlines every pass mode, keeps one copy to stay
 // within its registers).
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR,
           bool SPLIT = true>
 __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
                                           const int lane) {
-    constexpr bool PAIR = SWEEP && (SUB || ADD);
     uint32_t c[SIGMA];
     float r[SIGMA];
     uint32_t flags, seg0;
@@ -255,35 +289,19 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
     const uint32_t open =
         (reinterpret_cast<const uint32_t*>(a.tile_open)[tile >> 2] >> (8u * (tile & 3u))) & 0xffu;
 
-    // Every gather of the lane is issued before the tile's first store, so
-    // all of them are in flight together rather than one per element.
-    float xsv[SIGMA], xav[SIGMA], xv[SIGMA];
-#pragma unroll
-    for (int i = 0; i < SIGMA; ++i) {
-        if constexpr (PAIR) {
-            const FPair<TF> pr = gather_pair<TF>(ldsp, a.hot_p, a.xp, c[i]);
-            xsv[i] = SUB ? tof<TF>(pr.prev) : 0.0f;
-            xav[i] = tof<TF>(pr.cur);
-            xv[i] = XCUR ? xav[i] : gather<TF>(lds, a.hot_x, a.x, c[i]);
-        } else {
-            xsv[i] = SUB ? ldf<TF>(a.xs, c[i]) : 0.0f;
-            xav[i] = ADD ? ldf<TF>(a.xa, c[i]) : 0.0f;
-            xv[i] = SWEEP ? gather<TF>(lds, a.hot_x, a.x, c[i]) : 0.0f;
-        }
-    }
     // The tile's rows are consecutive unless it holds empty rows: two copies
     // of the rest, chosen by a wave-uniform branch, so the common one has no
     // segment-table loads (a load on only one of two merged paths makes the
     // compiler wait for every outstanding load at each later use).
     if constexpr (!SPLIT)
-        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, SEG_ANY>(a, tile, lane, flags, seg0, row0, segoff, open, c, r,
-                                                               xsv, xav, xv);
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_ANY>(a, lds, ldsp, tile, lane, flags, seg0, row0,
+                                                                     segoff, open, c, r);
     else if (segoff == 0xffffffffu)
-        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, SEG_RUN>(a, tile, lane, flags, seg0, row0, segoff, open, c, r,
-                                                               xsv, xav, xv);
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN>(a, lds, ldsp, tile, lane, flags, seg0, row0,
+                                                                     segoff, open, c, r);
     else
-        tile_rows<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, SEG_TABLE>(a, tile, lane, flags, seg0, row0, segoff, open, c,
-                                                                 r, xsv, xav, xv);
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_TABLE>(a, lds, ldsp, tile, lane, flags, seg0, row0,
+                                                                       segoff, open, c, r);
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], by one
