@@ -85,7 +85,7 @@ struct HiPlane<Idx24, SIGMA> {
 // The lane's streams of one tile, unpacked: SIGMA indices and values, its
 // row-start flags and first segment.  Indices are loaded first: the gathers
 // wait for them, not for the values.
-template <typename IdxT, int SIGMA>
+template <typename IdxT, int SIGMA, bool ORDERED>
 __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t tile, const int lane, uint32_t (&c)[SIGMA],
                                           float (&r)[SIGMA], uint32_t& flags, uint32_t& seg0) {
     using IE = typename IdxGroup<IdxT>::elem;
@@ -95,6 +95,9 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
     const float* __restrict__ tval = a.tval + (size_t)tile * TS;
     flags = a.lane_flags[(size_t)tile * WAVE + lane];
     seg0 = a.lane_seg0[(size_t)tile * WAVE + lane];
+    // ORDERED: keep these two loads ahead of the streams (the row lookups
+    // wait for them, and loads complete in issue order)
+    if constexpr (ORDERED) __builtin_amdgcn_sched_barrier(0);
     vu4 ci[SIGMA / G];
 #pragma unroll
     for (int q = 0; q < SIGMA / G; ++q)
@@ -205,13 +208,13 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
             g += (TA)xv[i] * (TA)v;
             h += (TA)xv[i] * (TA)xv[i];
         }
-    }
-    if (SUB || ADD) {
-        // padding positions of the last tile are rewritten too; they are never read as data
-#pragma unroll
-        for (int q = 0; q < SIGMA / 4; ++q)
-            __builtin_nontemporal_store(vf4{r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]},
-                                        reinterpret_cast<vf4*>(tval + q * 256 + 4 * lane));
+        // Each group of four updated values is stored as soon as it is done,
+        // so the stores are complete by the time the next tile's loads reuse
+        // their registers.  (Padding positions of the last tile are
+        // rewritten too; they are never read as data.)
+        if ((SUB || ADD) && (i & 3) == 3)
+            __builtin_nontemporal_store(vf4{r[i - 3], r[i - 2], r[i - 1], r[i]},
+                                        reinterpret_cast<vf4*>(tval + (i >> 2) * 256 + 4 * lane));
     }
     if (!SWEEP) return;
 
@@ -278,16 +281,26 @@ template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD
           bool SPLIT = true>
 __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
                                           const int lane) {
+    // Passes that update the residual look up per-row vectors first thing,
+    // so their tile descriptors and the lane's flags and first segment are
+    // loaded ahead of the streams: when the compiler makes the descriptors
+    // vector loads (it cannot always tell that the kernel's stores leave
+    // them alone), a wait for them must not also wait for the streams.
+    constexpr bool EARLY = SUB || ADD;
+    uint32_t row0 = 0, segoff = 0, open = 0;
+    auto descriptors = [&]() {
+        row0 = a.tile_row0[tile];
+        segoff = a.tile_segoff[tile];
+        // the byte of this tile, read as its 32-bit word (the array is padded
+        // to whole words) so that a wave-uniform tile can be a scalar load
+        open = (reinterpret_cast<const uint32_t*>(a.tile_open)[tile >> 2] >> (8u * (tile & 3u))) & 0xffu;
+    };
+    if constexpr (EARLY) descriptors();
     uint32_t c[SIGMA];
     float r[SIGMA];
     uint32_t flags, seg0;
-    load_tile<IdxT, SIGMA>(a, tile, lane, c, r, flags, seg0);
-    const uint32_t row0 = a.tile_row0[tile];
-    const uint32_t segoff = a.tile_segoff[tile];
-    // the byte of this tile, read as its 32-bit word (the array is padded to
-    // whole words) so that a wave-uniform tile gives a scalar load
-    const uint32_t open =
-        (reinterpret_cast<const uint32_t*>(a.tile_open)[tile >> 2] >> (8u * (tile & 3u))) & 0xffu;
+    load_tile<IdxT, SIGMA, EARLY>(a, tile, lane, c, r, flags, seg0);
+    if constexpr (!EARLY) descriptors();
 
     // The tile's rows are consecutive unless it holds empty rows: two copies
     // of the rest, chosen by a wave-uniform branch, so the common one has no
