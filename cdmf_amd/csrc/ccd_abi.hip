@@ -553,6 +553,7 @@ int32_t ccd_destroy(ccd_ctx* c) {
     dfree(c->v16);
     dfree(c->rmse_bsum);
     dfree(c->rmse_dev);
+    dfree(c->obj_dev);
     drop_graph(c);
     for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1018,6 +1019,21 @@ int32_t ccd_rmse(ccd_ctx* c, double* out) {
                                 c->rmse_bsum, c->rmse_dev, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(out, c->rmse_dev, sizeof(double), hipMemcpyDeviceToHost));
+    return CCD_OK;
+}
+
+int32_t ccd_objective(ccd_ctx* c, float lambda, double* out) {
+    if (!c || !out) return CCD_ERR_INVALID;
+    if (!c->factors) return set_err(c, CCD_ERR_STATE, "no factors");
+    if (!std::isfinite(lambda) || lambda < 0.0f) return set_err(c, CCD_ERR_INVALID, "lambda >= 0 finite");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->obj_dev) HIP_TRY(c, dalloc(&c->obj_dev, 3));
+    // the block partials fit the RMSE scratch (2 * OBJ_BLOCKS <= RMSE_BLOCKS doubles)
+    static_assert(2 * ccd::OBJ_BLOCKS <= ccd::RMSE_BLOCKS, "objective partials exceed the scratch");
+    HIP_TRY(c, ccd::launch_objective(c->csr.val, c->nnz, c->csr.ptr, c->csc.ptr, c->W, c->H, c->m, c->n, c->k,
+                                     (double)lambda, c->rmse_bsum, c->obj_dev, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->obj_dev, 3 * sizeof(double), hipMemcpyDeviceToHost));
     return CCD_OK;
 }
 

@@ -104,6 +104,7 @@ struct ccd_ctx {
     std::map<uint32_t, int> occupancy;  // persistent kernel -> blocks per CU by register use
     double* rmse_bsum = nullptr;
     double* rmse_dev = nullptr;  // one slot per outer iteration
+    double* obj_dev = nullptr;   // objective, loss, regulariser
     int32_t rmse_cap = 0;
     std::vector<double> iter_rmse;
     ccd_plan_params plan{4, 8192, 4096, 4096, 16, 0};

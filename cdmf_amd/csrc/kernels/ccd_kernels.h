@@ -157,6 +157,14 @@ constexpr int RMSE_BLOCKS = 1024;
 hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* tval, uint64_t nnz,
                        const void* W, const void* H, bool bf16_factors, int64_t m, int64_t n, uint32_t k,
                        double* block_sums, double* out, hipStream_t s);
+// Objective: out[0] = loss + lambda * reg, out[1] = loss = sum res^2, out[2]
+// = reg = sum_i |row i| ||w_i||^2 + sum_j |col j| ||h_j||^2, all in FP64,
+// reduced in a fixed order (block_sums: 2 * OBJ_BLOCKS doubles).  W [k][m],
+// H [k][n] FP32; row_ptr / col_ptr give the degrees.
+constexpr int OBJ_BLOCKS = 512;
+hipError_t launch_objective(const float* res, uint64_t nnz, const uint32_t* row_ptr, const uint32_t* col_ptr,
+                            const float* W, const float* H, int64_t m, int64_t n, uint32_t k, double lambda,
+                            double* block_sums, double* out, hipStream_t s);
 // End of a rank: dstS[i] = srcS[i] (i < nS) and dstF[j] = srcF[j] (j < nF) when
 // the sources are given, and the pair arrays of the next rank's fused passes
 //   pairS[i] = (prevS[i], nextS[i]),  pairF[j] = (prevF[j], nextF[j])

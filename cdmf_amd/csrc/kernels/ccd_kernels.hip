@@ -279,6 +279,77 @@ __global__ __launch_bounds__(BLOCK) void rmse_final_kernel(const double* __restr
     if (threadIdx.x == 0) out[0] = nnz ? sqrt(sh[0] / (double)nnz) : (double)NAN;
 }
 
+// The objective CCD++ minimises, in FP64: sum over the observed entries of
+// the residual squared, plus lambda * (sum_i |row i| ||w_i||^2 + sum_j
+// |col j| ||h_j||^2).  The reference computes the first sum in UpdateRating
+// and drops it (cdmf_ref.cpp:27,38).  Per-block partial sums in a fixed
+// order; term 0 is the loss, term 1 the regulariser.
+__global__ __launch_bounds__(BLOCK) void objective_partial_kernel(const float* __restrict__ res, uint64_t nnz,
+                                                                  const uint32_t* __restrict__ row_ptr,
+                                                                  const uint32_t* __restrict__ col_ptr,
+                                                                  const float* __restrict__ W,
+                                                                  const float* __restrict__ H, int64_t m, int64_t n,
+                                                                  uint32_t k, double* __restrict__ block_sums) {
+    double loss = 0.0, reg = 0.0;
+    const uint64_t tid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x, stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t e = tid; e < nnz; e += stride) loss += (double)res[e] * (double)res[e];
+    for (uint64_t i = tid; i < (uint64_t)(m + n); i += stride) {
+        const bool w = i < (uint64_t)m;
+        const uint64_t r = w ? i : i - (uint64_t)m;
+        const uint64_t dim = w ? (uint64_t)m : (uint64_t)n;
+        const float* X = w ? W : H;
+        const uint32_t* ptr = w ? row_ptr : col_ptr;
+        double sq = 0.0;
+        for (uint32_t t = 0; t < k; ++t) {
+            const double x = (double)X[(size_t)t * dim + r];
+            sq += x * x;
+        }
+        reg += (double)(ptr[r + 1] - ptr[r]) * sq;
+    }
+    __shared__ double sh[2][BLOCK];
+    sh[0][threadIdx.x] = loss;
+    sh[1][threadIdx.x] = reg;
+    __syncthreads();
+#pragma unroll
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        block_sums[2 * blockIdx.x] = sh[0][0];
+        block_sums[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void objective_final_kernel(const double* __restrict__ block_sums, int nb,
+                                                                double lambda, double* __restrict__ out) {
+    __shared__ double sh[2][BLOCK];
+    double loss = 0.0, reg = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLOCK) {
+        loss += block_sums[2 * b];
+        reg += block_sums[2 * b + 1];
+    }
+    sh[0][threadIdx.x] = loss;
+    sh[1][threadIdx.x] = reg;
+    __syncthreads();
+#pragma unroll
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0] + lambda * sh[1][0];
+        out[1] = sh[0][0];
+        out[2] = sh[1][0];
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
                                                      uint64_t count) {
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -404,6 +475,17 @@ hipError_t launch_rmse(const uint32_t* trow, const uint32_t* tcol, const float* 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rmse_final_kernel, dim3(1), dim3(BLOCK), 0, s, block_sums, RMSE_BLOCKS, nnz, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_objective(const float* res, uint64_t nnz, const uint32_t* row_ptr, const uint32_t* col_ptr,
+                            const float* W, const float* H, int64_t m, int64_t n, uint32_t k, double lambda,
+                            double* block_sums, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(objective_partial_kernel, dim3(OBJ_BLOCKS), dim3(BLOCK), 0, s, res, nnz, row_ptr, col_ptr, W, H,
+                       m, n, k, block_sums);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(objective_final_kernel, dim3(1), dim3(BLOCK), 0, s, block_sums, OBJ_BLOCKS, lambda, out);
     return hipGetLastError();
 }
 

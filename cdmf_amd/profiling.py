@@ -50,8 +50,8 @@ def kernel_class(name: str):
 
     sweep_kernel<IdxT, TF, TA, SUB, ADD, SWEEP> and
     tiled_kernel<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>: the first
-    template argument is the index type, the first three booleans are
-    SUB, ADD, SWEEP."""
+    template argument is the index type (unsigned short, ccd::Idx24 or
+    unsigned int), the first three booleans are SUB, ADD, SWEEP."""
     m = re.search(r"(sweep_kernel|tiled_kernel)<([^>]*)>", name)
     if not m:
         if "copy_kernel" in name:
@@ -70,7 +70,9 @@ def kernel_class(name: str):
     sub, add, sweep = bools[:3]
     if not sweep:
         return "residual"
-    layout = "f" if args[0] == "unsigned int" else "s"
+    # the first-swept layout (CSC) has 32-bit or 3-byte indices, the second
+    # (CSR of a matrix with at most 65,536 columns) 16-bit ones
+    layout = "s" if args[0] == "unsigned short" else "f"
     return ("fused_" if (sub or add) else "sweep_") + layout
 
 

@@ -72,6 +72,7 @@ def hip_lib() -> C.CDLL:
             "ccd_get_factors": ([vp, f32p, f32p], C.c_int32),
             "ccd_get_residual": ([vp, f32p, f32p], C.c_int32),
             "ccd_rmse": ([vp, f64p], C.c_int32),
+            "ccd_objective": ([vp, C.c_float, f64p], C.c_int32),
             "ccd_iter_rmse": ([vp, f64p, C.c_int32], C.c_int32),
             "ccd_kernel_stats": ([vp, C.c_int32, f64p, C.POINTER(C.c_uint64), f64p], C.c_int32),
             "ccd_loop_time": ([vp, f64p], C.c_int32),
@@ -100,7 +101,7 @@ def hip_lib() -> C.CDLL:
 ABI_SYMBOLS = (
     "ccd_abi_version", "ccd_device_count", "ccd_params_default", "ccd_create", "ccd_destroy", "ccd_last_error",
     "ccd_upload", "ccd_upload_test", "ccd_set_factors", "ccd_run", "ccd_get_factors", "ccd_get_residual",
-    "ccd_rmse", "ccd_iter_rmse", "ccd_kernel_stats", "ccd_loop_time", "ccd_copy_probe", "ccd_set_plan",
+    "ccd_rmse", "ccd_objective", "ccd_iter_rmse", "ccd_kernel_stats", "ccd_loop_time", "ccd_copy_probe", "ccd_set_plan",
     "ccd_batch_create", "ccd_batch_destroy", "ccd_batch_last_error", "ccd_batch_add", "ccd_batch_set_plan",
     "ccd_batch_get_plan", "ccd_batch_tune", "ccd_batch_run", "ccd_batch_time", "ccd_batch_teams",
 )
@@ -265,6 +266,13 @@ class Solver:
         out = C.c_double()
         self._check(self._lib.ccd_rmse(self._ctx, C.byref(out)))
         return out.value
+
+    def objective(self, lam: float):
+        """(objective, loss, regulariser) of the current state, on the device
+        (include/cdmf_hip.h, ccd_objective)."""
+        out = np.zeros(3, dtype=np.float64)
+        self._check(self._lib.ccd_objective(self._ctx, lam, N.ptr(out, C.c_double)))
+        return out[0], out[1], out[2]
 
     def kernel_stats(self) -> dict:
         out = {}

@@ -139,6 +139,16 @@ int32_t ccd_get_residual(ccd_ctx* ctx, float* csr_val, float* csc_val);
 /* Test RMSE of the current factors, FP64 reduction on the device. */
 int32_t ccd_rmse(ccd_ctx* ctx, double* out);
 
+/* The objective CCD++ minimises, for the current factors and residual, in
+ * FP64 on the device: out[0] = loss + lambda * reg, out[1] = loss = sum of
+ * the squared residual over the observed entries, out[2] = reg =
+ * sum_i |Omega_i| ||w_i||^2 + sum_j |Omega_j| ||h_j||^2.  The reference
+ * computes the loss inside UpdateRating and drops it (src/cdmf_ref.cpp:27,38);
+ * this is that reduction, fused over the residual the solver keeps.  Valid
+ * after ccd_run (which leaves the residual exact), or after ccd_set_factors,
+ * where the residual is R and W H^T is taken as 0. */
+int32_t ccd_objective(ccd_ctx* ctx, float lambda, double* out);
+
 /* Per-outer-iteration test RMSEs of the last ccd_run (count = maxiter). */
 int32_t ccd_iter_rmse(ccd_ctx* ctx, double* out, int32_t max_count);
 

@@ -218,6 +218,43 @@ def test_profile_kinds():
                 assert st["rmse"]["launches"] == outer
 
 
+@pytest.mark.parametrize("variant", [1, 3])
+def test_objective_on_device(oracle_lib, variant):
+    """ccd_objective: the FP64 reduction equals the oracle's objective of the
+    same state (W, H and residual read back from the GPU), is non-increasing
+    over outer iterations run one at a time (each run leaves the residual
+    exact, and a continued run repeats the arithmetic of one long run), and
+    ends within FP32 trajectory noise of the oracle's own solve."""
+    solver = _solver()
+    ds = _small()
+    k, lam, inner, outer = 8, 0.05, 3, 6
+    W0, H0 = oracle_lib.initial_col(k, ds.m), np.zeros((k, ds.n), np.float32)
+    objs = []
+    with solver.Solver(0) as s:
+        s.upload(ds)
+        s.set_factors(W0, H0)
+        lam32 = float(np.float32(lam))  # the ABI takes lambda as a float
+        o0 = s.objective(lam)
+        ref0 = oracle_lib.objective(ds, ds.csr_val, W0, H0, lam32)
+        _say("variant %d: objective at start %.10g (oracle %.10g)" % (variant, o0[0], ref0))
+        assert abs(o0[0] - ref0) <= 1e-10 * abs(ref0)
+        assert abs(o0[0] - (o0[1] + lam32 * o0[2])) <= 1e-12 * o0[0]
+        for it in range(outer):
+            s.run(k, lam, 1, inner, variant=variant, zero_first=(it == 0))
+            o = s.objective(lam)
+            W, H = s.factors()
+            rcsr, _ = s.residual()
+            ref = oracle_lib.objective(ds, rcsr, W, H, lam32)
+            assert abs(o[0] - ref) <= 1e-10 * abs(ref), (it, o[0], ref)
+            objs.append(o[0])
+    _say("objective per outer iteration: %s" % ["%.8g" % x for x in objs])
+    for a, b in zip(objs, objs[1:]):
+        assert b <= a * (1 + 1e-6)
+    Wr, Hr, _, rr, _ = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner)
+    ref_final = oracle_lib.objective(ds, rr, Wr, Hr, lam)
+    assert abs(objs[-1] - ref_final) <= 1e-4 * ref_final
+
+
 def test_rmse_kernel_matches_oracle(oracle_lib):
     solver = _solver()
     ds = _small()
