@@ -89,6 +89,7 @@ ccd::BatchLayout batch_layout(const ccdi::DevLayout& L) {
     d.nnz = L.nnz;
     d.ntiles = T.ntiles;
     d.nspan = T.nspan;
+    d.nshort = T.nshort;
     d.ib = (uint32_t)T.ib;
     d.rows = (uint32_t)L.rows;
     return d;

@@ -390,7 +390,10 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
     }
     T.sigma = sigma;
     T.ntiles = (uint32_t)ntiles;
+    // two-tile spans first (finished one per lane), the rest after them
+    std::stable_partition(span.begin(), span.end(), [](const ccd::SpanRow& q) { return q.t1 == q.t0 + 1; });
     T.nspan = (uint32_t)span.size();
+    T.nshort = (uint32_t)std::count_if(span.begin(), span.end(), [](const ccd::SpanRow& q) { return q.t1 == q.t0 + 1; });
     int rc;
     if ((rc = upload_vec(c, &T.flags, flags)) || (rc = upload_vec(c, &T.seg0, seg0)) ||
         (rc = upload_vec(c, &T.row0, row0)) || (rc = upload_vec(c, &T.segoff, segoff)) ||
@@ -427,7 +430,7 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
-        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, lambda};
+        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, T.nshort, lambda};
         HIP_TRY(c, ccd::launch_span(sa, c->prec, c->stream));
     }
     return CCD_OK;

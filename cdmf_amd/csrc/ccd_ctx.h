@@ -28,8 +28,9 @@ struct TiledLayout {
     uint8_t* open = nullptr;
     double2* head = nullptr;  // (g, h) pairs; double-sized so every precision fits
     double2* tail = nullptr;
-    ccd::SpanRow* spans = nullptr;  // rows spanning tiles
+    ccd::SpanRow* spans = nullptr;  // rows spanning tiles: two-tile ones first
     uint32_t nspan = 0;
+    uint32_t nshort = 0;
 };
 
 struct DevLayout {

@@ -191,8 +191,9 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
         tiles<SIGMA>(&L, pv, sub, add, true, w, nw);
         if (!team_sync(tm)) return false;
         if (L.nspan) {
-            const SpanArgs sa{L.spans, L.head, L.tail, pv.y, L.nspan, lambda};
-            for (uint32_t i = w; i < L.nspan; i += nw) tiled::span_row<float, float>(sa, i, lane);
+            const SpanArgs sa{L.spans, L.head, L.tail, pv.y, L.nspan, L.nshort, lambda};
+            for (uint32_t i = tid; i < L.nshort; i += nth) tiled::span_short<float, float>(sa, i);
+            for (uint32_t i = L.nshort + w; i < L.nspan; i += nw) tiled::span_row<float, float>(sa, i, lane);
             if (!team_sync(tm)) return false;
         }
         return true;

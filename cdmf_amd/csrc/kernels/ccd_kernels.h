@@ -265,12 +265,15 @@ struct SpanRow {
     uint32_t row, t0, t1, len;
 };
 
+// spans[0, nshort) cover two tiles (t1 = t0 + 1) and are finished one per
+// lane; spans[nshort, nspan) cover more and are finished one per wave.
 struct SpanArgs {
     const SpanRow* spans;
     const void* head;
     const void* tail;
     void* y;
     uint32_t nspan;
+    uint32_t nshort;
     float lambda;
 };
 
@@ -323,7 +326,8 @@ struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
     uint64_t nnz;
     uint32_t ntiles;
     uint32_t nspan;
-    uint32_t ib;  // bytes per index of the tiled copy: 2, 3 or 4
+    uint32_t nshort;  // spans over two tiles, first in the span table
+    uint32_t ib;      // bytes per index of the tiled copy: 2, 3 or 4
     uint32_t rows;
 };
 

@@ -317,6 +317,19 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
                                                                        segoff, open, c, r);
 }
 
+// A row spanning two tiles, by one lane: tail[t0] + head[t0 + 1], rounded
+// exactly as span_row rounds it (the head first passes through a sum with
+// zero, as in span_row's tree).
+template <typename TF, typename TA>
+__device__ __forceinline__ void span_short(const SpanArgs& a, uint32_t i) {
+    const SpanRow sp = a.spans[i];
+    const Pair<TA> tl = static_cast<const Pair<TA>*>(a.tail)[sp.t0];
+    const Pair<TA> hd = static_cast<const Pair<TA>*>(a.head)[sp.t1];
+    const TA g = tl.g + ((TA)0 + hd.g);
+    const TA h = tl.h + ((TA)0 + hd.h);
+    stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
+}
+
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], by one
 // wave, lane-strided partial sums reduced in a fixed tree.  The span table
 // holds the tile range and length, so the partials are the only dependent

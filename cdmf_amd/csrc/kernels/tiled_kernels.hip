@@ -51,9 +51,17 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
         tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane);
 }
 
+// Blocks [0, ceil(nshort / BLOCK)) finish two-tile spans, one per thread;
+// the rest finish longer spans, one per wave.
 template <typename TF, typename TA>
 __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
-    const uint32_t i = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const uint32_t sb = (a.nshort + BLOCK - 1) / BLOCK;
+    if (blockIdx.x < sb) {
+        const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+        if (i < a.nshort) tiled::span_short<TF, TA>(a, i);
+        return;
+    }
+    const uint32_t i = a.nshort + (blockIdx.x - sb) * WPB + (threadIdx.x >> 6);
     if (i >= a.nspan) return;
     tiled::span_row<TF, TA>(a, i, threadIdx.x & (WAVE - 1));
 }
@@ -194,7 +202,8 @@ int tiled_blocks_per_cu(int sigma, int prec, int ib, bool sub, bool add, bool sw
 
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
     if (a.nspan == 0) return hipSuccess;
-    const dim3 grid((a.nspan + WPB - 1) / WPB);
+    if (a.nshort > a.nspan) return hipErrorInvalidValue;
+    const dim3 grid((a.nshort + BLOCK - 1) / BLOCK + (a.nspan - a.nshort + WPB - 1) / WPB);
     switch (prec) {
         case PREC_F32: hipLaunchKernelGGL((tiled_span_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
         case PREC_BF16: hipLaunchKernelGGL((tiled_span_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
