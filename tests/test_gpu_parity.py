@@ -384,6 +384,60 @@ def test_tiled_variant(oracle_lib, sigma):
         assert np.all(got[1][:, empty] == 0.0)  # empty columns -> 0 (the reference's V3 would give NaN)
 
 
+def _tall(m, n, active, per_row, seed):
+    """m rows of which `active` hold ratings (per_row random columns each, at
+    most): the CSC copy gathers over all m rows, which picks the tiled index
+    format (3 bytes above 2^16 rows, 4 above 2^24)."""
+    from cdmf_amd import datadir
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(m, size=active, replace=False)).astype(np.int64)
+    r = np.repeat(rows, per_row)
+    c = rng.integers(0, n, size=r.size)
+    key = np.unique(r * n + c)  # one rating per (row, column)
+    r, c = key // n, key % n
+    v = rng.integers(1, 6, size=r.size).astype(np.float32)
+    test = rng.choice(r.size, size=r.size // 10, replace=False)
+    return datadir.from_triplets(m, n, (r, c, v), (r[test], c[test], v[test]))
+
+
+def _compare_tall(ds, got, ref, label):
+    """Rows that hold ratings against the oracle (the empty rows' zeros would
+    shrink the elementwise floor to nothing; their W is exactly 0 in both):
+    RMSE per outer iteration within 1e-4, and W, H and the residual to the
+    full-size bar of _close_bulk with 99.99 % of the entries inside the
+    elementwise bound.  Rows with a handful of ratings amplify FP32
+    summation-order noise, as at full size."""
+    active = np.diff(ds.csr_ptr.astype(np.int64)) > 0
+    assert np.all(got[0][:, ~active] == 0.0) and np.all(ref[0][:, ~active] == 0.0)
+    np.testing.assert_allclose(got[2], ref[2], rtol=0, atol=RMSE_ATOL)
+    _close_bulk(got[0][:, active], ref[0][:, active], label + " W", quantile=0.9999)
+    _close_bulk(got[1], ref[1], label + " H", quantile=0.9999)
+    _close_bulk(got[3], ref[3], label + " residual CSR", quantile=0.9999)
+    _close_bulk(got[4], ref[4], label + " residual CSC", quantile=0.9999)
+
+
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_tiled_3byte_indices(oracle_lib, sigma):
+    ds = _tall((1 << 16) + 5000, 400, 30000, 12, seed=11)
+    k, lam = 4, 0.05
+    _say("case tiled 3-byte indices sigma=%d: m=%d nnz=%d" % (sigma, ds.m, ds.nnz))
+    W0, H0 = oracle_lib.initial_col(k, ds.m), np.zeros((k, ds.n), np.float32)
+    got = _run_gpu(ds, W0, H0, k, lam, 2, 2, variant=3, plan=dict(sigma=sigma))
+    ref = oracle_lib.ccd(ds, W0, H0, k, lam, 2, 2)
+    _compare_tall(ds, got, ref, "tiled 3-byte s=%d" % sigma)
+
+
+@pytest.mark.skipif(os.environ.get("CDMF_SKIP_LARGE") == "1", reason="large case disabled by CDMF_SKIP_LARGE=1")
+def test_tiled_32bit_indices(oracle_lib):
+    ds = _tall((1 << 24) + 1000, 300, 40000, 8, seed=12)
+    k, lam = 3, 0.05
+    _say("case tiled 32-bit indices: m=%d nnz=%d" % (ds.m, ds.nnz))
+    W0, H0 = oracle_lib.initial_col(k, ds.m), np.zeros((k, ds.n), np.float32)
+    got = _run_gpu(ds, W0, H0, k, lam, 2, 2, variant=3)
+    ref = oracle_lib.ccd(ds, W0, H0, k, lam, 2, 2)
+    _compare_tall(ds, got, ref, "tiled 32-bit")
+
+
 def test_tiled_wfirst_and_switching(oracle_lib):
     """W-first order on the tiled path, then a binned run continuing from the
     tiled residual: the residual hand-over between layouts is exact."""
