@@ -30,8 +30,7 @@ struct Pair {
 };
 
 // y_row = g / (lambda |row| + h); the row's length d comes from the tile's
-// row-start flags, not from the row pointer, so no load waits behind the
-// prefetched streams of the next tile.
+// row-start flags, not from two row-pointer loads per finished row.
 template <typename TF, typename TA>
 __device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA g, TA h, uint32_t d) {
     stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));

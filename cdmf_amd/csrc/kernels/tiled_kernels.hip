@@ -14,16 +14,18 @@
 // l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of
 // G = 16 / (bytes per entry), element i of lane l at (i/G)*64G + G*l + i%G,
 // so every 16-byte load of a lane moves 1 KiB of consecutive values or
-// indices per wave (G = 4 for values and 32-bit indices, 8 for 16-bit ones).  The
+// indices per wave (G = 4 for values and 32-bit indices, 8 for 16-bit ones;
+// 3-byte indices add a plane of their high bytes, see ccd_kernels.h).  The
 // streams are read and written with non-temporal hints so that they do not
 // evict the gathered factor vector from L2.  Each lane updates its elements
-// and keeps a running (g, h) for the row it is in.  A row that starts and ends inside a
-// lane is finished by that lane.  Rows crossing lane boundaries are combined
-// by a segmented inclusive scan over the 64 lanes (shuffles, fixed order);
-// rows crossing tile boundaries leave per-tile partials (head/tail) that
-// tiled_span_kernel adds up in tile order.  No atomics: results are
-// run-to-run deterministic.  Built with -ffp-contract=off like the binned
-// kernels.
+// and keeps a running (g, h) for the row it is in.  A row that starts and
+// ends inside a lane is finished by that lane.  Rows crossing lane boundaries
+// are combined by a segmented inclusive scan over the 64 lanes (shuffles,
+// fixed order); rows crossing tile boundaries leave per-tile partials
+// (head/tail) that tiled_span_kernel adds up in tile order.  A row's length
+// comes from the row-start flags (its start and end positions in the tile).
+// No atomics: results are run-to-run deterministic.  Built with
+// -ffp-contract=off like the binned kernels.
 #include "tile_body.h"
 
 namespace ccd {
