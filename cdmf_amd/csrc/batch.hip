@@ -21,6 +21,8 @@ struct ccd_batch {
     std::vector<Item> items;
     ccd_batch_plan plan{16, 0, 1.0f, 0};
     int ncu = 0;
+    int lds_cu = 0, lds_block = 0;  // LDS bytes of a CU and of a block
+    uint32_t lds_floats = 0;        // dynamic LDS of a block of the last grid_blocks, in 4-byte entries
     // device copies of the launch description
     ccd::BatchProblem* d_probs = nullptr;
     uint32_t* d_block_problem = nullptr;
@@ -92,6 +94,8 @@ ccd::BatchLayout batch_layout(const ccdi::DevLayout& L) {
     d.nshort = T.nshort;
     d.ib = (uint32_t)T.ib;
     d.rows = (uint32_t)L.rows;
+    d.gdim = (uint32_t)L.gdim;
+    d.stage = 0;
     return d;
 }
 
@@ -190,6 +194,10 @@ int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool tri
         ccd::BatchProblem& P = probs[i];
         P.F = batch_layout(first_layout(c, it.p));
         P.S = batch_layout(second_layout(c, it.p));
+        // stage a layout's gathered vector when each block streams at least
+        // twice as many elements as it would stage
+        for (ccd::BatchLayout* L : {&P.F, &P.S})
+            L->stage = L->nnz / std::max(nb[i], 1u) >= 2ull * std::min<uint64_t>(L->gdim, b->lds_floats) ? 1u : 0u;
         P.XF = hfirst ? c->H : c->W;
         P.XS = hfirst ? c->W : c->H;
         P.vF = hfirst ? c->v : c->u;
@@ -235,6 +243,7 @@ int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool tri
     args->status = b->d_status;
     args->wait_ticks = 100000000ull * 4;  // 4 s of the 100 MHz wall clock
     args->trial = trial ? 1u : 0u;
+    args->lds_floats = b->lds_floats;
     return CCD_OK;
 }
 
@@ -273,9 +282,17 @@ int enqueue_run(ccd_batch* b, int sigma, const ccd::BatchArgs& a, bool trial) {
 }
 
 // Grid of the persistent launch: co-resident blocks, a multiple of 8.
+// Each block's dynamic LDS (staged hot prefixes) is its share of the CU's
+// LDS at the occupancy its registers allow, so staging never lowers the
+// number of co-resident blocks; sets b->lds_floats.
 int grid_blocks(ccd_batch* b, int sigma, uint32_t want, uint32_t* out) {
-    const int bpc = ccd::batch_blocks_per_cu(sigma);
+    const int regs = ccd::batch_blocks_per_cu(sigma, 0);
+    if (regs <= 0) return berr(b, CCD_ERR_HIP, "occupancy query of the batched kernel failed");
+    const int share = std::min(b->lds_block, b->lds_cu / regs) - 1024;  // 1 KiB for the kernel's own LDS
+    const size_t lds = share > 0 ? (size_t)share & ~(size_t)255 : 0;
+    const int bpc = ccd::batch_blocks_per_cu(sigma, lds);
     if (bpc <= 0) return berr(b, CCD_ERR_HIP, "occupancy query of the batched kernel failed");
+    b->lds_floats = (uint32_t)(lds / sizeof(float));
     uint32_t most = (uint32_t)(bpc * b->ncu) & ~7u;
     uint32_t g = want ? std::min(want, most) & ~7u : most;
     if (g < b->items.size())
@@ -331,6 +348,10 @@ int32_t ccd_batch_create(int32_t device, ccd_batch** out) {
     }
     if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || b->ncu <= 0)
         b->ncu = 1;
+    if (hipDeviceGetAttribute(&b->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess)
+        b->lds_cu = 0;
+    if (hipDeviceGetAttribute(&b->lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+        b->lds_block = 0;
     *out = b;
     return CCD_OK;
 }

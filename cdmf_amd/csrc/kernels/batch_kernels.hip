@@ -105,29 +105,41 @@ __device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const Pas
     a.tail = L.tail;
     a.nnz = L.nnz;
     a.ntiles = L.ntiles;
-    a.hot_x = 0;  // no LDS staging: a team's gathered vectors stay in its XCD's L2
+    a.hot_x = 0;  // set per pass by the caller (LDS staging)
     a.hot_p = 0;
     a.x_is_cur = v.xcur;
     a.lambda = v.lambda;
     return a;
 }
 
+// What a block staged in its LDS for one pass: the first hot_x entries of
+// the gathered vector x, or the first hot_p (previous, current) pairs.
+struct Staged {
+    const float* lds;
+    const FPair<float>* ldsp;
+    uint32_t hot_x, hot_p;
+};
+
 // The tiles of one pass taken by wave w of nw.
 template <int SIGMA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
-__device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, uint32_t w, uint32_t nw) {
+__device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, const Staged& st, uint32_t w, uint32_t nw) {
     const BatchLayout& L = *Lp;
-    const TiledArgs a = layout_args(L, v);
+    TiledArgs a = layout_args(L, v);
+    a.hot_x = st.hot_x;
+    a.hot_p = st.hot_p;
+    const float* lds = st.lds;
+    const FPair<float>* ldsp = st.ldsp;
     const int lane = threadIdx.x & (WAVE - 1);
     w = __builtin_amdgcn_readfirstlane(w);  // wave-uniform: scalar loads of the tile descriptors
     if (L.ib == 2) {
         for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, nullptr, nullptr, t, lane);
+            tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
     } else if (L.ib == 3) {
         for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<Idx24, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, nullptr, nullptr, t, lane);
+            tiled::tile_body<Idx24, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
     } else {
         for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<uint32_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, nullptr, nullptr, t, lane);
+            tiled::tile_body<uint32_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
     }
 }
 
@@ -138,23 +150,46 @@ __device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, uint3
 // as the fused add-only pass of the single-problem path, which is left out
 // here because it makes the compiler spill the whole kernel.)
 template <int SIGMA>
-__device__ __forceinline__ void tiles(const BatchLayout* L, const PassVecs& v, bool sub, bool add, bool sweep,
-                                      uint32_t w, uint32_t nw) {
+__device__ __forceinline__ void tiles(const BatchLayout* L, const PassVecs& v, const Staged& st, bool sub, bool add,
+                                      bool sweep, uint32_t w, uint32_t nw) {
     const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0) | (sweep && (sub || add) && v.xcur ? 8 : 0);
     switch (code) {
-        case 1: tiles_t<SIGMA, false, false, true, false>(L, v, w, nw); break;
-        case 2: tiles_t<SIGMA, false, true, false, false>(L, v, w, nw); break;
-        case 4: tiles_t<SIGMA, true, false, false, false>(L, v, w, nw); break;
-        case 5: tiles_t<SIGMA, true, false, true, false>(L, v, w, nw); break;
-        case 7: tiles_t<SIGMA, true, true, true, false>(L, v, w, nw); break;
-        case 8 + 5: tiles_t<SIGMA, true, false, true, true>(L, v, w, nw); break;
-        case 8 + 7: tiles_t<SIGMA, true, true, true, true>(L, v, w, nw); break;
+        case 1: tiles_t<SIGMA, false, false, true, false>(L, v, st, w, nw); break;
+        case 2: tiles_t<SIGMA, false, true, false, false>(L, v, st, w, nw); break;
+        case 4: tiles_t<SIGMA, true, false, false, false>(L, v, st, w, nw); break;
+        case 5: tiles_t<SIGMA, true, false, true, false>(L, v, st, w, nw); break;
+        case 7: tiles_t<SIGMA, true, true, true, false>(L, v, st, w, nw); break;
+        case 8 + 5: tiles_t<SIGMA, true, false, true, true>(L, v, st, w, nw); break;
+        case 8 + 7: tiles_t<SIGMA, true, true, true, true>(L, v, st, w, nw); break;
         default: break;  // not produced by the loop below
     }
 }
 
+// Stage the hot prefix a sweep pass gathers (the caller has passed a
+// barrier or __syncthreads since the LDS was last read): x for plain
+// sweeps and for fused passes with a separate x, the pairs for fused passes
+// whose x is the pair's current value.  Empty when the layout does not
+// stage.
+__device__ __forceinline__ Staged stage_pass(const BatchLayout& L, const PassVecs& v, bool fused, float* lds,
+                                             uint32_t cap) {
+    Staged st{lds, reinterpret_cast<const FPair<float>*>(lds), 0u, 0u};
+    if (!L.stage) return st;
+    if (fused && v.xcur) {
+        st.hot_p = min(L.gdim, cap / 2);
+        const FPair<float>* src = static_cast<const FPair<float>*>(L.pair);
+        FPair<float>* dst = reinterpret_cast<FPair<float>*>(lds);
+        for (uint32_t i = threadIdx.x; i < st.hot_p; i += BATCH_BLOCK) dst[i] = src[i];
+    } else {
+        st.hot_x = min(L.gdim, cap);
+        for (uint32_t i = threadIdx.x; i < st.hot_x; i += BATCH_BLOCK) lds[i] = v.x[i];
+    }
+    __syncthreads();
+    return st;
+}
+
 template <int SIGMA>
 __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
+    extern __shared__ __align__(16) float batch_lds[];
     // Virtual block id: the XCD this block runs on (hardware XCC id
     // register) and a ticket among that XCD's blocks.  Workgroups are dealt
     // to the 8 XCDs round-robin, so each XCD receives nblocks / 8 of them
@@ -186,9 +221,14 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
     const uint32_t dF = P.F.rows, dS = P.S.rows;
     const float lambda = P.lambda;
 
-    // one sweep (or fused) pass: tiles, barrier, rows spanning tiles, barrier
+    const Staged none{nullptr, nullptr, 0u, 0u};
+
+    // one sweep (or fused) pass: LDS staging, tiles, barrier, rows spanning
+    // tiles, barrier
     auto pass = [&](const BatchLayout& L, const PassVecs& pv, bool sub, bool add) -> bool {
-        tiles<SIGMA>(&L, pv, sub, add, true, w, nw);
+        __syncthreads();  // every wave of the block is done with the LDS of the last pass
+        const Staged st = stage_pass(L, pv, sub || add, batch_lds, ba.lds_floats);
+        tiles<SIGMA>(&L, pv, st, sub, add, true, w, nw);
         if (!team_sync(tm)) return false;
         if (L.nspan) {
             const SpanArgs sa{L.spans, L.head, L.tail, pv.y, L.nspan, L.nshort, lambda};
@@ -222,10 +262,10 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
             const float* Fp = P.XF + (size_t)prev * dF;
             const bool add = oiter > 1 || P.continued;
             if (add && !pending) {  // rank 0 of a continued run: add back, then plain sweeps
-                tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, nullptr, nullptr, St, Ft, 0u, lambda}, false, true, false,
-                             w, nw);
-                tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, nullptr, nullptr, Ft, St, 0u, lambda}, false, true, false,
-                             w, nw);
+                tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, nullptr, nullptr, St, Ft, 0u, lambda}, none, false, true,
+                             false, w, nw);
+                tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, nullptr, nullptr, Ft, St, 0u, lambda}, none, false, true,
+                             false, w, nw);
                 // each wave sweeps the tiles it just updated: no barrier needed
                 if (!pass(P.F, plain(St, P.vF), false, false)) return;
                 if (!pass(P.S, plain(P.vF, P.vS), false, false)) return;
@@ -251,8 +291,10 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
     if (pending) {  // rank prev's final subtraction; the two layouts are independent
         const float* Sp = P.XS + (size_t)prev * dS;
         const float* Fp = P.XF + (size_t)prev * dF;
-        tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, Sp, Fp, nullptr, nullptr, 0u, lambda}, true, false, false, w, nw);
-        tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, Fp, Sp, nullptr, nullptr, 0u, lambda}, true, false, false, w, nw);
+        tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, Sp, Fp, nullptr, nullptr, 0u, lambda}, none, true, false, false,
+                     w, nw);
+        tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, Fp, Sp, nullptr, nullptr, 0u, lambda}, none, true, false, false,
+                     w, nw);
     }
 }
 
@@ -260,19 +302,20 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
 
 hipError_t launch_batch(const BatchArgs& a, int sigma, hipStream_t s) {
     if (a.nblocks == 0 || a.nblocks % 8u) return hipErrorInvalidValue;
+    const size_t lds = (size_t)a.lds_floats * sizeof(float);
     switch (sigma) {
-        case 8: hipLaunchKernelGGL(batch_kernel<8>, dim3(a.nblocks), dim3(BATCH_BLOCK), 0, s, a); break;
-        case 16: hipLaunchKernelGGL(batch_kernel<16>, dim3(a.nblocks), dim3(BATCH_BLOCK), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(batch_kernel<8>, dim3(a.nblocks), dim3(BATCH_BLOCK), lds, s, a); break;
+        case 16: hipLaunchKernelGGL(batch_kernel<16>, dim3(a.nblocks), dim3(BATCH_BLOCK), lds, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-int batch_blocks_per_cu(int sigma) {
+int batch_blocks_per_cu(int sigma, size_t lds_bytes) {
     int nb = 0;
     hipError_t e = hipErrorInvalidValue;
-    if (sigma == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<8>, BATCH_BLOCK, 0);
-    if (sigma == 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<16>, BATCH_BLOCK, 0);
+    if (sigma == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<8>, BATCH_BLOCK, lds_bytes);
+    if (sigma == 16) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<16>, BATCH_BLOCK, lds_bytes);
     return e == hipSuccess && nb > 0 ? nb : 0;
 }
 

@@ -329,6 +329,8 @@ struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
     uint32_t nshort;  // spans over two tiles, first in the span table
     uint32_t ib;      // bytes per index of the tiled copy: 2, 3 or 4
     uint32_t rows;
+    uint32_t gdim;    // length of the gathered vector
+    uint32_t stage;   // 1: each block stages the gathered vector's hot prefix in LDS per pass
 };
 
 struct BatchProblem {
@@ -354,11 +356,13 @@ struct BatchArgs {
     int* status;                    // 0; 1 after a barrier wait ran out
     uint64_t wait_ticks;            // bound of one barrier wait in wall_clock64() ticks
     uint32_t trial;                 // autotuner trial: trial_reps plain sweep pairs per problem, nothing else
+    uint32_t lds_floats;            // dynamic LDS of a block, in 4-byte entries (staged hot prefixes)
 };
 
-// sigma 8 or 16
+// sigma 8 or 16; dynamic LDS of a.lds_floats entries per block
 hipError_t launch_batch(const BatchArgs& a, int sigma, hipStream_t s);
-int batch_blocks_per_cu(int sigma);
+// co-resident blocks per CU with lds_bytes of dynamic LDS each (0 on failure)
+int batch_blocks_per_cu(int sigma, size_t lds_bytes);
 
 // ---------------------------------------------------------------- fold-in ---
 // Rows (CSR, the new users' ratings) folded in against fixed item factors:

@@ -113,6 +113,23 @@ def test_batch_bitwise_equals_single_runs(sigma):
     b.close()
 
 
+def test_batch_small_grid_stages_in_lds():
+    """16 blocks for 5 problems (two per XCD): every team streams far more
+    elements per block than it would stage, so each pass stages the gathered
+    vector's hot prefix (or the pairs) in LDS; results stay bit-identical to
+    single-problem runs."""
+    S = _solver()
+    cases = _cases()
+    b, solvers = _build(S, cases, 16)
+    b.set_plan(sigma=16, blocks=16)
+    t = b.run()
+    _say("batch run on 16 blocks %.3f ms, teams %s" % (1e3 * t, b.teams()))
+    for (name, ds, k, lam, outer, inner, order, W0, H0), g in zip(cases, _batch_results(solvers)):
+        _say("compare %s" % name)
+        _same(g, _single(S, ds, k, lam, outer, inner, order, W0, H0, 16), name)
+    b.close()
+
+
 def test_batch_continued_and_graph_replay():
     """Second batch run continues the first (add-back from rank 0); graph
     replays after ccd_set_factors repeat the first run exactly."""
