@@ -289,7 +289,9 @@ int grid_blocks(ccd_batch* b, int sigma, uint32_t want, uint32_t* out) {
     const int regs = ccd::batch_blocks_per_cu(sigma, 0);
     if (regs <= 0) return berr(b, CCD_ERR_HIP, "occupancy query of the batched kernel failed");
     const int share = std::min(b->lds_block, b->lds_cu / regs) - 1024;  // 1 KiB for the kernel's own LDS
-    const size_t lds = share > 0 ? (size_t)share & ~(size_t)255 : 0;
+    // at least 256 bytes: fused passes read entry 0 of the staged area for
+    // lanes past a (possibly empty) staged prefix
+    const size_t lds = share > 256 ? (size_t)share & ~(size_t)255 : 256;
     const int bpc = ccd::batch_blocks_per_cu(sigma, lds);
     if (bpc <= 0) return berr(b, CCD_ERR_HIP, "occupancy query of the batched kernel failed");
     b->lds_floats = (uint32_t)(lds / sizeof(float));

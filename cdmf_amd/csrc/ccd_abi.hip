@@ -19,18 +19,26 @@
 // upload (stable, so ties keep their order).  The highest-degree rows and
 // columns -- the entries gathered most often -- become a contiguous prefix
 // of every factor vector, which the sweeps stage in LDS, and the sweep work
-// units become contiguous row ranges.  Element order inside a row or column
-// is kept, so every per-row sum adds the same terms in the same order as
-// without renumbering.  W, H, the test set and the residual are mapped at
-// the boundary; callers only ever see their own numbering.
+// units become contiguous row ranges.  When the whole gathered vector of a
+// layout fits the LDS prefix of a plain sweep, the elements of each row are
+// sorted by internal index as well: the fused passes, which split their LDS
+// between two gathered arrays, then find most tiles of the tiled copy
+// gathering only from LDS.  (For a larger gathered vector the sort would
+// pack the gathers beyond the prefix into the row ends, which measured
+// slower, so element order is kept there.)  Sorting reorders the terms of
+// each per-row sum, which are trees anyway, not the reference's serial
+// order.  W, H, the test set and the residual are mapped at the boundary;
+// callers only ever see their own numbering.
 #include "ccd_ctx.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ccdi {
@@ -53,6 +61,7 @@ void free_tiled(TiledLayout& T) {
     dfree(T.segoff);
     dfree(T.segrows);
     dfree(T.open);
+    dfree(T.gmax);
     dfree(T.head);
     dfree(T.tail);
     dfree(T.spans);
@@ -402,6 +411,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
         return rc;
     const uint64_t padded = ntiles * TS;
     HIP_TRY(c, dalloc(&T.tval, padded));
+    HIP_TRY(c, dalloc(&T.gmax, ntiles));
     HIP_TRY(c, dalloc(&T.head, ntiles));
     HIP_TRY(c, dalloc(&T.tail, ntiles));
     T.ib = tiled_index_bytes(L);
@@ -410,7 +420,10 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
         HIP_TRY(c, dalloc(&d, padded * (size_t)T.ib));
         T.tidx = d;
     }
-    if (ntiles) HIP_TRY(c, ccd::launch_retile_idx(L.idx, L.idx16, T.tidx, T.ib, nnz, T.ntiles, sigma, c->stream));
+    if (ntiles) {
+        HIP_TRY(c, ccd::launch_retile_idx(L.idx, L.idx16, T.tidx, T.ib, nnz, T.ntiles, sigma, c->stream));
+        HIP_TRY(c, ccd::launch_tile_gmax(L.idx, L.idx16, T.segoff, T.gmax, nnz, T.ntiles, sigma, c->stream));
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return CCD_OK;
 }
@@ -422,7 +435,7 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     const Launch sh = launch_shape(c, true, T.sigma, T.ib, sub, add, sweep, fused && x_is_cur);
     const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
-                     T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.head, T.tail, L.nnz, T.ntiles,
+                     T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.gmax, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
     {
         Timed tm(c, kind, true);
@@ -446,23 +459,45 @@ std::vector<uint32_t> degree_order(const uint32_t* ptr, int64_t rows) {
 }
 
 // One compressed layout in internal numbering: rows in `perm` order, indices
-// mapped through `ipos` (caller's index -> internal), element order inside
-// each row kept.
+// mapped through `ipos` (caller's index -> internal), and, with `sort`, the
+// elements of each row sorted by internal index (equal indices keep their
+// order; without it the caller's order is kept).  src[o]: the caller's
+// position of internal element o.  Rows are processed in parallel, in
+// chunks taken from a shared counter.
 void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
               const std::vector<uint32_t>& perm, const std::vector<uint32_t>& ipos, std::vector<uint32_t>& optr,
-              std::vector<uint32_t>& oidx, std::vector<float>& oval) {
+              std::vector<uint32_t>& oidx, std::vector<float>& oval, std::vector<uint32_t>& src, bool sort) {
     optr.assign((size_t)rows + 1, 0u);
     for (int64_t i = 0; i < rows; ++i) optr[i + 1] = optr[i] + (ptr[perm[i] + 1] - ptr[perm[i]]);
     oidx.resize(optr[rows]);
     oval.resize(optr[rows]);
-    for (int64_t i = 0; i < rows; ++i) {
-        const uint32_t r = perm[i];
-        uint32_t o = optr[i];
-        for (uint32_t e = ptr[r]; e < ptr[r + 1]; ++e, ++o) {
-            oidx[o] = ipos[idx[e]];
-            oval[o] = val[e];
+    src.resize(optr[rows]);
+    constexpr int64_t CHUNK = 512;
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        std::vector<uint64_t> key;
+        for (int64_t b; (b = next.fetch_add(CHUNK)) < rows;) {
+            for (int64_t i = b; i < std::min(rows, b + CHUNK); ++i) {
+                const uint32_t r = perm[i], e0 = ptr[r], len = ptr[r + 1] - ptr[r];
+                key.resize(len);
+                for (uint32_t q = 0; q < len; ++q) key[q] = (uint64_t)ipos[idx[e0 + q]] << 32 | q;
+                if (sort) std::sort(key.begin(), key.end());
+                const uint32_t o = optr[i];
+                for (uint32_t q = 0; q < len; ++q) {
+                    const uint32_t e = e0 + (uint32_t)key[q];
+                    oidx[o + q] = (uint32_t)(key[q] >> 32);
+                    oval[o + q] = val[e];
+                    src[o + q] = e;
+                }
+            }
         }
-    }
+    };
+    const uint64_t nnz = optr[rows];
+    const unsigned nt = nnz < (1u << 20) ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
 }
 
 const char* check_compressed(int64_t rows, int64_t dim, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
@@ -603,16 +638,20 @@ int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_
         std::vector<uint32_t> rpos((size_t)m), cpos((size_t)n);
         for (int64_t i = 0; i < m; ++i) rpos[rperm[i]] = (uint32_t)i;
         for (int64_t j = 0; j < n; ++j) cpos[cperm[j]] = (uint32_t)j;
-        std::vector<uint32_t> p2, i2;
+        std::vector<uint32_t> p2, i2, src;
         std::vector<float> v2;
-        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2);
+        // Rows are sorted by gathered index only when the whole gathered
+        // vector fits the LDS prefix of a plain sweep (FP32 entries of one
+        // block per CU, as launch_shape sizes it).
+        const int64_t lds_entries = std::max(0, std::min(c->lds_block, c->lds_cu) - 1024) / 4;
+        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2, src, n <= lds_entries);
         int rc = upload_layout(c, c->csr, m, n, nnz, p2.data(), i2.data(), v2.data());
         if (rc) return rc;
-        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2);
+        c->csr.src = std::move(src);
+        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2, src, m <= lds_entries);
         rc = upload_layout(c, c->csc, n, m, nnz, p2.data(), i2.data(), v2.data());
         if (rc) return rc;
-        c->csr.ext_ptr.assign(row_ptr, row_ptr + m + 1);
-        c->csc.ext_ptr.assign(col_ptr, col_ptr + n + 1);
+        c->csc.src = std::move(src);
         c->csr.perm = std::move(rperm);
         c->csc.perm = std::move(cperm);
     }
@@ -1000,11 +1039,7 @@ int32_t ccd_get_residual(ccd_ctx* c, float* csr_val, float* csc_val) {
         const DevLayout& L = q == 0 ? c->csr : c->csc;
         if (!out) continue;
         HIP_TRY(c, hipMemcpy(tmp.data(), L.val, 4 * c->nnz, hipMemcpyDeviceToHost));
-        // internal row i holds the caller's row perm[i], in the caller's element order
-        for (int64_t i = 0; i < L.rows; ++i) {
-            const uint32_t r = L.perm[i];
-            std::memcpy(out + L.ext_ptr[r], tmp.data() + L.hptr[i], 4 * (size_t)(L.hptr[i + 1] - L.hptr[i]));
-        }
+        for (uint64_t e = 0; e < c->nnz; ++e) out[L.src[e]] = tmp[e];
     }
     return CCD_OK;
 }

@@ -26,6 +26,7 @@ struct TiledLayout {
     uint32_t* segoff = nullptr;
     uint32_t* segrows = nullptr;
     uint8_t* open = nullptr;
+    uint32_t* gmax = nullptr;  // largest gathered index per tile
     double2* head = nullptr;  // (g, h) pairs; double-sized so every precision fits
     double2* tail = nullptr;
     ccd::SpanRow* spans = nullptr;  // rows spanning tiles: two-tile ones first
@@ -60,7 +61,7 @@ struct DevLayout {
     uint64_t split_cap = 0;
     uint32_t split_x = 0, split_p = 0;
     std::vector<uint32_t> hptr;      // host copy of the internal ptr (planning)
-    std::vector<uint32_t> ext_ptr;   // the caller's ptr (residual download)
+    std::vector<uint32_t> src;       // caller's position of each internal element (residual download)
     std::vector<uint32_t> perm;      // internal row -> caller's row
     TiledLayout tl;
 };

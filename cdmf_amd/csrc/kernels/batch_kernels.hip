@@ -101,6 +101,7 @@ __device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const Pas
     a.tile_segoff = L.tile_segoff;
     a.segrows = L.segrows;
     a.tile_open = L.tile_open;
+    a.tile_gmax = nullptr;  // one tile_core copy here: every tile takes the general path
     a.head = L.head;
     a.tail = L.tail;
     a.nnz = L.nnz;

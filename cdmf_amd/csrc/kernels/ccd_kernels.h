@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ccd {
 
 // (previous rank's final value, current rank's value) of one factor entry,
@@ -24,6 +26,7 @@ __device__ __forceinline__ FPair<TF> gather_pair(const FPair<TF>* lds, uint32_t 
 }
 
 // Gather of a factor value: from the LDS-staged hot prefix when c < hot.
+// (The compiler makes this one flat load through the selected pointer.)
 template <class TF>
 __device__ __forceinline__ float gather(const TF* lds, uint32_t hot, const void* g, uint32_t c) {
     if (c < hot) {
@@ -32,6 +35,36 @@ __device__ __forceinline__ float gather(const TF* lds, uint32_t hot, const void*
     }
     if constexpr (sizeof(TF) == 2) return __uint_as_float((uint32_t) static_cast<const uint16_t*>(g)[c] << 16);
     else return static_cast<const float*>(g)[c];
+}
+
+// Pointers of a fixed address space (LDS, global).  The fused tiled passes
+// read LDS and global memory through these, so the compiler cannot turn a
+// choice between the two into one flat load through a selected pointer
+// (which waits on both the LDS and the memory counters).  gfx950's compiler
+// also rejects a flat select between an LDS and a global pointer in a kernel
+// that reads LDS through lds_ptr ("Operand has incorrect register class"),
+// so a kernel uses one form or the other.
+template <class T>
+using lds_ptr = const __attribute__((address_space(3))) T*;
+template <class T>
+using glb_ptr = const __attribute__((address_space(1))) T*;
+
+// Read of an array that no kernel writes while the reader runs (the tiled
+// layout's per-tile descriptors), through the constant address space: at a
+// wave-uniform index it is a scalar load, whatever stores the kernel makes.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p, size_t i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// A pair is read as one word (8 bytes for FP32, 4 for BF16) and split.
+template <class TF>
+using PairWord = typename std::conditional<sizeof(TF) == 4, uint64_t, uint32_t>::type;
+template <class TF>
+__device__ __forceinline__ FPair<TF> to_pair(PairWord<TF> w) {
+    FPair<TF> p;
+    __builtin_memcpy(&p, &w, sizeof(p));
+    return p;
 }
 
 // Native clang vector types: the non-temporal load/store builtins take these
@@ -132,7 +165,9 @@ struct FinalizeArgs {
     uint32_t nsplit;
 };
 
-// Dynamic LDS of a persistent sweep: hot_x factor entries, then hot_p pairs.
+// Dynamic LDS of a persistent sweep: hot_x factor entries, then hot_p pairs,
+// then 16 bytes that the fused tiled passes read (and drop) for lanes past
+// the prefixes.
 size_t staged_lds_bytes(int prec, uint32_t hot_x, uint32_t hot_p);
 
 // Optional timing of one launch: events bound to the kernel's own dispatch
@@ -229,6 +264,9 @@ hipError_t launch_copy(const float4* src, float4* dst, uint64_t count, hipStream
 //               into segrows (row of every segment of the tile)
 //   tile_open   bit 0: the tile's first row started in an earlier tile;
 //               bit 1: its last row continues into the next tile
+//   tile_gmax   largest gathered index of the tile (0xffffffff for a last
+//               tile with padding and for tiles with empty rows): a sweep
+//               whose staged prefix covers it gathers the whole tile from LDS
 // Rows that span tiles are finished by tiled_span_kernel from the per-tile
 // partial sums: tail[t0] (the row's part in its first tile) plus head[t] for
 // every later tile t of the row.
@@ -249,6 +287,7 @@ struct TiledArgs {
     const uint32_t* tile_segoff;
     const uint32_t* segrows;
     const uint8_t* tile_open;
+    const uint32_t* tile_gmax;
     void* head;  // (g, h) pairs of the accumulator type
     void* tail;
     uint64_t nnz;
@@ -295,6 +334,11 @@ hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_
 // u16 or u32 indices in, the tiled copy in the format of ib out
 hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uint64_t nnz, uint32_t ntiles, int sigma,
                              hipStream_t s);
+// gmax[t] = largest of the indices of tile t (original order), 0xffffffff
+// for a tile that extends past nnz or holds empty rows (segoff[t] other
+// than 0xffffffff)
+hipError_t launch_tile_gmax(const void* idx, bool idx16, const uint32_t* segoff, uint32_t* gmax, uint64_t nnz,
+                            uint32_t ntiles, int sigma, hipStream_t s);
 hipError_t launch_fill(float* dst, float value, uint64_t count, hipStream_t s);
 hipError_t launch_fill_bytes16(uint16_t* dst, uint16_t value, uint64_t count, hipStream_t s);
 

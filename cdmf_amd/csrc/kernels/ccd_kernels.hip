@@ -416,7 +416,7 @@ SweepFn sweep_fn(int prec, bool idx16, bool sub, bool add, bool sweep) {
 
 size_t staged_lds_bytes(int prec, uint32_t hot_x, uint32_t hot_p) {
     const size_t es = prec == PREC_BF16 ? 2 : 4;
-    return (((size_t)hot_x * es + 15) & ~(size_t)15) + (size_t)hot_p * 2 * es;
+    return (((size_t)hot_x * es + 15) & ~(size_t)15) + (size_t)hot_p * 2 * es + 16;
 }
 
 hipError_t record_unlaunched(const LaunchEvents* ev, hipStream_t s) {

@@ -83,8 +83,10 @@ struct HiPlane<Idx24, SIGMA> {
 
 // The lane's streams of one tile, unpacked: SIGMA indices and values, its
 // row-start flags and first segment.  Indices are loaded first: the gathers
-// wait for them, not for the values.
-template <typename IdxT, int SIGMA, bool ORDERED>
+// wait for them, not for the values.  HI: the high bytes of the 3-byte
+// format are loaded and added; a tile whose indices are all below 2^16
+// leaves them out.
+template <typename IdxT, int SIGMA, bool ORDERED, bool HI = true>
 __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t tile, const int lane, uint32_t (&c)[SIGMA],
                                           float (&r)[SIGMA], uint32_t& flags, uint32_t& seg0) {
     using IE = typename IdxGroup<IdxT>::elem;
@@ -102,7 +104,7 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
     for (int q = 0; q < SIGMA / G; ++q)
         ci[q] = __builtin_nontemporal_load(reinterpret_cast<const vu4*>(tidx + q * G * WAVE + G * lane));
     HiPlane<IdxT, SIGMA> hi;
-    hi.load(a, tile, lane);
+    if constexpr (HI) hi.load(a, tile, lane);
 #pragma unroll
     for (int q = 0; q < SIGMA / 4; ++q) {
         const vf4 rv = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(tval + q * 256 + 4 * lane));
@@ -116,7 +118,8 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
         const uint32_t wd[4] = {ci[q].x, ci[q].y, ci[q].z, ci[q].w};
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-            if constexpr (G == 8) c[8 * q + j] = ((wd[j >> 1] >> (16 * (j & 1))) & 0xffffu) | hi.at(8 * q + j);
+            if constexpr (G == 8)
+                c[8 * q + j] = ((wd[j >> 1] >> (16 * (j & 1))) & 0xffffu) | (HI ? hi.at(8 * q + j) : 0u);
             else c[4 * q + j] = wd[j];
         }
     }
@@ -125,13 +128,18 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
 // One tile after its streams are loaded: the lane's per-row vectors and
 // gathers, the residual update, per-row sums and row finishes.  SEG: how the
 // tile's rows are found (seg_row).  XCUR (fused passes only): the sweep's x
-// is the pair's current value.
-template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR, int SEG>
+// is the pair's current value.  HOT (fused passes only): every gather of the
+// tile is below the staged counts and the tile has no padding (it is not a
+// partial last tile), so the gathers are plain LDS reads and no element is
+// masked.
+template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR, int SEG,
+          bool HOT = false>
 __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
                                           const int lane, const uint32_t flags, const uint32_t seg0,
                                           const uint32_t row0, const uint32_t segoff, const uint32_t open,
                                           const uint32_t (&c)[SIGMA], float (&r)[SIGMA]) {
     constexpr bool PAIR = SWEEP && (SUB || ADD);
+    static_assert(PAIR || !HOT, "only fused passes have an LDS-only path");
     constexpr uint32_t TS = WAVE * SIGMA;
     float* __restrict__ tval = a.tval + (size_t)tile * TS;
     const uint64_t lbase = (uint64_t)tile * TS + (uint64_t)lane * SIGMA;
@@ -153,18 +161,53 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
 
     // Every gather of the lane is issued before the tile's first store, so
     // all of them are in flight together rather than one per element.
+    // Fused passes: HOT tiles read LDS only; other tiles read LDS for every
+    // element (entry 0 past the staged prefix), then overwrite the elements
+    // past it with global loads.  All LDS reads are issued first, so the
+    // global loads wait for them once rather than one by one (a global load
+    // into a register must wait for the LDS read it overwrites).  Plain
+    // sweeps gather with one flat load per element (gather): with the
+    // unsorted rows of a gathered vector larger than the LDS prefix, this
+    // measured faster than separate LDS and global loads.
     float xsv[SIGMA], xav[SIGMA], xv[SIGMA];
+    const lds_ptr<TF> l3 = (lds_ptr<TF>)lds;
+    const lds_ptr<PairWord<TF>> p3 = (lds_ptr<PairWord<TF>>)ldsp;
+    if constexpr (PAIR) {
+        PairWord<TF> pr[SIGMA];
+        TF xg[SIGMA];
 #pragma unroll
-    for (int i = 0; i < SIGMA; ++i) {
-        if constexpr (PAIR) {
-            const FPair<TF> pr = gather_pair<TF>(ldsp, a.hot_p, a.xp, c[i]);
-            xsv[i] = SUB ? tof<TF>(pr.prev) : 0.0f;
-            xav[i] = tof<TF>(pr.cur);
-            xv[i] = XCUR ? xav[i] : gather<TF>(lds, a.hot_x, a.x, c[i]);
-        } else {
+        for (int i = 0; i < SIGMA; ++i) {
+            pr[i] = p3[HOT || c[i] < a.hot_p ? c[i] : 0u];
+            if (!XCUR) xg[i] = l3[HOT || c[i] < a.hot_x ? c[i] : 0u];
+        }
+        if constexpr (!HOT) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < SIGMA; ++i) {
+                if (c[i] >= a.hot_p) pr[i] = ((glb_ptr<PairWord<TF>>)a.xp)[c[i]];
+                if (!XCUR && c[i] >= a.hot_x) xg[i] = ((glb_ptr<TF>)a.x)[c[i]];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) {
+            const FPair<TF> p = to_pair<TF>(pr[i]);
+            xsv[i] = SUB ? tof<TF>(p.prev) : 0.0f;
+            xav[i] = tof<TF>(p.cur);
+            xv[i] = XCUR ? xav[i] : tof<TF>(xg[i]);
+        }
+    } else if constexpr (SWEEP) {
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) {
+            xsv[i] = 0.0f;
+            xav[i] = 0.0f;
+            xv[i] = gather<TF>(lds, a.hot_x, a.x, c[i]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) {
             xsv[i] = SUB ? ldf<TF>(a.xs, c[i]) : 0.0f;
             xav[i] = ADD ? ldf<TF>(a.xa, c[i]) : 0.0f;
-            xv[i] = SWEEP ? gather<TF>(lds, a.hot_x, a.x, c[i]) : 0.0f;
+            xv[i] = 0.0f;
         }
     }
 
@@ -198,7 +241,7 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
                 if (ADD) yar = ldf<TF>(a.ya, row);
             }
         }
-        const bool valid = lbase + (uint64_t)i < a.nnz;
+        const bool valid = HOT || lbase + (uint64_t)i < a.nnz;
         float v = r[i];
         if (SUB) v = v - xsv[i] * ysr;
         if (ADD) v = v + xav[i] * yar;
@@ -272,39 +315,67 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
     }
 }
 
+// Largest gathered index for which a tile of a fused pass is HOT: the
+// staged count of everything the pass gathers (the pairs, and x unless it
+// is the pair's current value), at most 2^16 for 3-byte indices (HOT tiles
+// skip the high bytes).
+template <typename IdxT, bool XCUR>
+__device__ __forceinline__ uint32_t hot_limit(const TiledArgs& a) {
+    uint32_t lim;
+    if constexpr (XCUR) lim = a.hot_p;
+    else lim = a.hot_p < a.hot_x ? a.hot_p : a.hot_x;
+    if constexpr (std::is_same<IdxT, Idx24>::value) lim = lim < 0x10000u ? lim : 0x10000u;
+    return lim;
+}
+
 // One tile: streams, then tile_core.  `tile` is wave-uniform, so the
-// per-tile descriptors are scalar loads.  SPLIT: separate code for tiles with and without empty rows (faster; the
-// batched kernel, which inlines every pass mode, keeps one copy to stay
-// within its registers).
+// per-tile descriptors are scalar loads.  SPLIT: separate code for fused-
+// pass tiles whose gathers all hit LDS (common when each row's elements are
+// sorted by gathered index), for the other tiles with consecutive rows and
+// for tiles with empty rows (faster; the batched kernel, which inlines every
+// pass mode, keeps one copy to stay within its registers).  gmax: the tile's
+// tile_gmax entry (fused passes), loaded by the caller one tile ahead so
+// that the choice is made before the tile's streams are issued (each copy
+// issues its own).
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR,
           bool SPLIT = true>
 __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
-                                          const int lane) {
+                                          const int lane, const uint32_t gmax = 0xffffffffu) {
     // Passes that update the residual look up per-row vectors first thing,
     // so their tile descriptors and the lane's flags and first segment are
-    // loaded ahead of the streams: when the compiler makes the descriptors
-    // vector loads (it cannot always tell that the kernel's stores leave
-    // them alone), a wait for them must not also wait for the streams.
+    // loaded ahead of the streams: a wait for them must not also wait for
+    // the streams.
     constexpr bool EARLY = SUB || ADD;
     uint32_t row0 = 0, segoff = 0, open = 0;
     auto descriptors = [&]() {
-        row0 = a.tile_row0[tile];
-        segoff = a.tile_segoff[tile];
+        row0 = ld_const(a.tile_row0, tile);
+        segoff = ld_const(a.tile_segoff, tile);
         // the byte of this tile, read as its 32-bit word (the array is padded
         // to whole words) so that a wave-uniform tile can be a scalar load
-        open = (reinterpret_cast<const uint32_t*>(a.tile_open)[tile >> 2] >> (8u * (tile & 3u))) & 0xffu;
+        open = (ld_const(reinterpret_cast<const uint32_t*>(a.tile_open), tile >> 2) >> (8u * (tile & 3u))) & 0xffu;
     };
-    if constexpr (EARLY) descriptors();
     uint32_t c[SIGMA];
     float r[SIGMA];
     uint32_t flags, seg0;
+
+    // A fused-pass tile with gmax below the staged counts has consecutive
+    // rows (tiles with empty rows carry 0xffffffff) and gathers only from
+    // LDS.  The other tiles' rows are consecutive unless the tile holds empty
+    // rows.  Copies of the rest are chosen by wave-uniform branches, so the
+    // common one has no segment-table loads (a load on only one of two merged
+    // paths makes the compiler wait for every outstanding load at each later
+    // use), no global gathers and, for 3-byte indices, no high bytes.
+    constexpr bool PAIR = SWEEP && (SUB || ADD);
+    if (SPLIT && PAIR && gmax < hot_limit<IdxT, XCUR>(a)) {
+        descriptors();
+        load_tile<IdxT, SIGMA, true, false>(a, tile, lane, c, r, flags, seg0);
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN, PAIR>(a, lds, ldsp, tile, lane, flags, seg0,
+                                                                           row0, segoff, open, c, r);
+        return;
+    }
+    if constexpr (EARLY) descriptors();
     load_tile<IdxT, SIGMA, EARLY>(a, tile, lane, c, r, flags, seg0);
     if constexpr (!EARLY) descriptors();
-
-    // The tile's rows are consecutive unless it holds empty rows: two copies
-    // of the rest, chosen by a wave-uniform branch, so the common one has no
-    // segment-table loads (a load on only one of two merged paths makes the
-    // compiler wait for every outstanding load at each later use).
     if constexpr (!SPLIT)
         tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_ANY>(a, lds, ldsp, tile, lane, flags, seg0, row0,
                                                                      segoff, open, c, r);

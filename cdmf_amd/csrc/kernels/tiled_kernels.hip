@@ -49,8 +49,18 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     const uint32_t stride = gridDim.x * NW;
     // wave-uniform tile index: the per-tile descriptors become scalar loads
     const uint32_t first = __builtin_amdgcn_readfirstlane(blockIdx.x * NW + (threadIdx.x >> 6));
-    for (uint32_t tile = first; tile < a.ntiles; tile += stride)
-        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane);
+    // Fused passes: each tile's tile_gmax entry is loaded while the previous
+    // tile runs (unconditionally, at a clamped tile index: a load on only one
+    // of two merged paths would make the compiler drain every outstanding
+    // load at the end of each tile).
+    constexpr bool PAIR = SWEEP && (SUB || ADD);
+    uint32_t gmax = PAIR && first < a.ntiles ? ld_const(a.tile_gmax, first) : 0xffffffffu;
+    for (uint32_t tile = first; tile < a.ntiles; tile += stride) {
+        const uint32_t next = tile + stride < a.ntiles ? tile + stride : tile;
+        const uint32_t gnext = PAIR ? ld_const(a.tile_gmax, next) : 0xffffffffu;
+        tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane, gmax);
+        gmax = gnext;
+    }
 }
 
 // Blocks [0, ceil(nshort / BLOCK)) finish two-tile spans, one per thread;
@@ -109,6 +119,32 @@ __global__ __launch_bounds__(BLOCK) void retile_idx24_kernel(const uint32_t* __r
         const uint32_t v = e < nnz ? src[e] : 0u;
         lo[tile * ts + (i >> 3) * (8 * WAVE) + 8 * l + (i & 7)] = (uint16_t)(v & 0xffffu);
         hi[e] = (uint8_t)(v >> 16);  // e = tile * ts + l * sigma + i
+    }
+}
+
+// Largest index of each tile, one wave per tile (grid-stride over tiles);
+// a tile that extends past nnz (the padded last one) or holds empty rows
+// (segoff other than 0xffffffff) gets 0xffffffff.
+template <typename IdxT>
+__global__ __launch_bounds__(BLOCK) void tile_gmax_kernel(const IdxT* __restrict__ idx,
+                                                          const uint32_t* __restrict__ segoff,
+                                                          uint32_t* __restrict__ gmax, uint64_t nnz, uint32_t ntiles,
+                                                          uint32_t sigma) {
+    const uint64_t ts = (uint64_t)WAVE * sigma;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (uint32_t t = blockIdx.x * WPB + (threadIdx.x >> 6); t < ntiles; t += gridDim.x * WPB) {
+        const uint64_t base = (uint64_t)t * ts;
+        uint32_t mx = 0;
+        for (uint64_t e = base + lane; e < base + ts && e < nnz; e += WAVE) {
+            const uint32_t v = idx[e];
+            mx = v > mx ? v : mx;
+        }
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_xor(mx, off);
+            mx = o > mx ? o : mx;
+        }
+        if (lane == 0) gmax[t] = base + ts > nnz || segoff[t] != 0xffffffffu ? 0xffffffffu : mx;
     }
 }
 
@@ -241,6 +277,20 @@ hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uin
     } else {
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_gmax(const void* idx, bool idx16, const uint32_t* segoff, uint32_t* gmax, uint64_t nnz,
+                            uint32_t ntiles, int sigma, hipStream_t s) {
+    if (ntiles == 0) return hipSuccess;
+    uint32_t blocks = (ntiles + WPB - 1) / WPB;
+    if (blocks > 16384) blocks = 16384;
+    if (idx16)
+        hipLaunchKernelGGL(tile_gmax_kernel<uint16_t>, dim3(blocks), dim3(BLOCK), 0, s,
+                           static_cast<const uint16_t*>(idx), segoff, gmax, nnz, ntiles, (uint32_t)sigma);
+    else
+        hipLaunchKernelGGL(tile_gmax_kernel<uint32_t>, dim3(blocks), dim3(BLOCK), 0, s,
+                           static_cast<const uint32_t*>(idx), segoff, gmax, nnz, ntiles, (uint32_t)sigma);
     return hipGetLastError();
 }
 
