@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="the 2,000 x 1,000 case instead (quick check)")
     ap.add_argument("--variant", type=int, default=DEFAULT_VARIANT, help="1/2 = degree-binned, 3 = tiled sweeps")
     ap.add_argument("--sigma", type=int, default=16, help="elements per lane of a tile (tiled sweeps)")
+    ap.add_argument("--split-rows", type=int, default=1, choices=(0, 1),
+                    help="tiled sweeps: keep the elements gathered from beyond the LDS prefix in tiles of their own")
     ap.add_argument("--blocks-per-cu", type=int, default=0,
                     help="persistent sweep blocks per CU (0: as many as the kernel's registers allow)")
     ap.add_argument("--batch", action="store_true",
@@ -134,7 +136,7 @@ def main():
     H0 = np.zeros((k, ds.n), np.float32)
 
     s = S.Solver(0)
-    s.set_plan(sigma=args.sigma, blocks_per_cu=args.blocks_per_cu)
+    s.set_plan(sigma=args.sigma, blocks_per_cu=args.blocks_per_cu, split_rows=args.split_rows)
     if args.probe:
         ms = s.copy_probe(256 << 20, reps=5)
         log("copy probe: 256 MiB in %.1f us (%.0f GB/s read+write)" % (ms * 1e3, 2 * (256 << 20) / (ms * 1e-3) / 1e9))
@@ -240,7 +242,7 @@ def bench_batch(args):
     for bp in specs:
         ds = D.batch_dataset(bp)
         s = S.Solver(0)
-        s.set_plan(sigma=args.sigma)
+        s.set_plan(sigma=args.sigma, split_rows=0)  # the batched kernel's plain tile layouts
         s.upload(ds)
         W0 = D.initial_col(bp.solve.k, ds.m)
         H0 = np.zeros((bp.solve.k, ds.n), np.float32)

@@ -65,6 +65,10 @@ void free_tiled(TiledLayout& T) {
     dfree(T.head);
     dfree(T.tail);
     dfree(T.spans);
+    dfree(T.vpos);
+    dfree(T.cold_row);
+    dfree(T.comb);
+    dfree(T.part);
     T = TiledLayout{};
 }
 
@@ -182,12 +186,20 @@ struct Timed {
 // Algorithmic HBM bytes of one pass over layout L: the index and value
 // streams (values read, and written back when the residual is updated), the
 // row pointer and the row list, one read of every distinct gathered vector
-// and per-row vector, and the output vector.
+// and per-row vector, and the output vector.  In a split tiled layout with
+// 3-byte indices the first parts' elements (indices below the split, under
+// 2^16) sit in tiles that read only the 16-bit plane: 2 bytes each.
 int tiled_index_bytes(const DevLayout& L) { return L.gdim <= 65536 ? 2 : L.gdim <= (1 << 24) ? 3 : 4; }
 
-double layout_bytes(const DevLayout& L, bool tiled, bool residual_write, int gathered, int per_row, bool writes_y) {
+double layout_bytes(const DevLayout& L, bool tiled, bool split, bool residual_write, int gathered, int per_row,
+                    bool writes_y) {
     const double ib = tiled ? (double)tiled_index_bytes(L) : L.idx16 ? 2.0 : 4.0;
     double b = (double)L.nnz * (ib + 4.0 + (residual_write ? 4.0 : 0.0));
+    if (tiled && split && L.split_at && ib == 3.0) {
+        uint64_t nb = 0;
+        for (uint32_t v : L.below) nb += v;
+        b -= (double)nb;
+    }
     b += 4.0 * (double)(L.rows + 1) + 4.0 * (double)L.rows;
     b += 4.0 * (double)L.gdim * gathered + 4.0 * (double)L.rows * per_row;
     if (writes_y) b += 4.0 * (double)L.rows;
@@ -348,15 +360,48 @@ int upload_vec(ccd_ctx* c, T** dst, const std::vector<T>& v) {
 
 // Tile descriptors from the row pointer (host, O(rows + tiles)), index array
 // tiled on the device.
-int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
-    if (L.tl.sigma == sigma) return CCD_OK;
+// split: with rows split at L.split_at, keep each row's two parts as two
+// virtual rows, all first parts (rows in order) before all second parts,
+// so that a tile gathers either from the LDS prefix only or from beyond it
+// only (the batched kernel asks for the plain layout).
+int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
+    split = split && L.split_at != 0;
+    if (L.tl.sigma == sigma && L.tl.split == split) return CCD_OK;
     free_tiled(L.tl);
     TiledLayout& T = L.tl;
     const uint64_t TS = 64ull * (uint64_t)sigma, nnz = L.nnz;
     const uint64_t ntiles = (nnz + TS - 1) / TS;
     if (ntiles >= 0xffffffffull / 64) return set_err(c, CCD_ERR_INVALID, "too many tiles");
-    const uint32_t* ptr = L.hptr.data();
-    const int64_t rows = L.rows;
+    // virtual rows of a split layout: row pointer, the layout position of
+    // every element in tiled order, the real row of every second part and
+    // each row's {second part, first part's length}
+    std::vector<uint32_t> vptr, vpos, cold_row;
+    std::vector<uint2> comb;
+    if (split) {
+        const int64_t nr = L.rows;
+        const uint32_t* hp = L.hptr.data();
+        vptr.assign(1, 0u);
+        vptr.reserve((size_t)nr * 2 + 1);
+        comb.resize((size_t)nr);
+        for (int64_t r = 0; r < nr; ++r) vptr.push_back(vptr.back() + L.below[r]);
+        for (int64_t r = 0; r < nr; ++r) {
+            const uint32_t rest = hp[r + 1] - hp[r] - L.below[r];
+            comb[r] = make_uint2(rest ? (uint32_t)(nr + cold_row.size()) : 0xffffffffu, L.below[r]);
+            if (rest) {
+                cold_row.push_back((uint32_t)r);
+                vptr.push_back(vptr.back() + rest);
+            }
+        }
+        vpos.resize(nnz);
+        for (int64_t r = 0; r < nr; ++r)
+            for (uint32_t q = 0; q < L.below[r]; ++q) vpos[vptr[r] + q] = hp[r] + q;
+        for (size_t k = 0; k < cold_row.size(); ++k) {
+            const uint32_t r = cold_row[k], b = vptr[nr + k];
+            for (uint32_t q = 0; q < vptr[nr + k + 1] - b; ++q) vpos[b + q] = hp[r] + L.below[r] + q;
+        }
+    }
+    const uint32_t* ptr = split ? vptr.data() : L.hptr.data();
+    const int64_t rows = split ? (int64_t)vptr.size() - 1 : L.rows;
     std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows;
     std::vector<ccd::SpanRow> span;
     std::vector<uint16_t> seg0(ntiles * 64);
@@ -398,6 +443,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
         }
     }
     T.sigma = sigma;
+    T.split = split;
     T.ntiles = (uint32_t)ntiles;
     // two-tile spans first (finished one per lane), the rest after them
     std::stable_partition(span.begin(), span.end(), [](const ccd::SpanRow& q) { return q.t1 == q.t0 + 1; });
@@ -409,6 +455,13 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
         (rc = upload_vec(c, &T.segrows, segrows)) || (rc = upload_vec(c, &T.open, open)) ||
         (rc = upload_vec(c, &T.spans, span)))
         return rc;
+    if (split) {
+        T.nreal = (uint32_t)L.rows;
+        if ((rc = upload_vec(c, &T.vpos, vpos)) || (rc = upload_vec(c, &T.cold_row, cold_row)) ||
+            (rc = upload_vec(c, &T.comb, comb)))
+            return rc;
+        HIP_TRY(c, dalloc(&T.part, (size_t)rows));
+    }
     const uint64_t padded = ntiles * TS;
     HIP_TRY(c, dalloc(&T.tval, padded));
     HIP_TRY(c, dalloc(&T.gmax, ntiles));
@@ -421,8 +474,8 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma) {
         T.tidx = d;
     }
     if (ntiles) {
-        HIP_TRY(c, ccd::launch_retile_idx(L.idx, L.idx16, T.tidx, T.ib, nnz, T.ntiles, sigma, c->stream));
-        HIP_TRY(c, ccd::launch_tile_gmax(L.idx, L.idx16, T.segoff, T.gmax, nnz, T.ntiles, sigma, c->stream));
+        HIP_TRY(c, ccd::launch_retile_idx(L.idx, L.idx16, T.tidx, T.ib, nnz, T.ntiles, sigma, T.vpos, c->stream));
+        HIP_TRY(c, ccd::launch_tile_gmax(L.idx, L.idx16, T.segoff, T.gmax, nnz, T.ntiles, sigma, T.vpos, c->stream));
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return CCD_OK;
@@ -436,15 +489,21 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.gmax, T.head, T.tail, L.nnz, T.ntiles,
-                     hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda};
+                     hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda, T.cold_row, T.nreal,
+                     sweep ? T.part : nullptr};
     {
         Timed tm(c, kind, true);
         HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, T.ib, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
     }
     if (sweep && T.nspan) {
         Timed tm(c, CCD_K_FINALIZE);
-        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, T.nshort, lambda};
+        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, T.nshort, lambda, T.part};
         HIP_TRY(c, ccd::launch_span(sa, c->prec, c->stream));
+    }
+    if (sweep && T.split) {
+        Timed tm(c, CCD_K_FINALIZE);
+        const ccd::CombineArgs ca{T.part, T.comb, L.ptr, y, lambda, T.nreal};
+        HIP_TRY(c, ccd::launch_combine(ca, c->prec, c->stream));
     }
     return CCD_OK;
 }
@@ -458,20 +517,27 @@ std::vector<uint32_t> degree_order(const uint32_t* ptr, int64_t rows) {
     return perm;
 }
 
+// Element order inside each row of a renumbered layout: the caller's, sorted
+// by internal index, or split into the elements whose internal index is
+// below a bound followed by the others (each part in the caller's order).
+enum class RowOrder { keep, sort, split };
+
 // One compressed layout in internal numbering: rows in `perm` order, indices
-// mapped through `ipos` (caller's index -> internal), and, with `sort`, the
-// elements of each row sorted by internal index (equal indices keep their
-// order; without it the caller's order is kept).  src[o]: the caller's
-// position of internal element o.  Rows are processed in parallel, in
-// chunks taken from a shared counter.
+// mapped through `ipos` (caller's index -> internal), elements inside each
+// row in `order` (split at internal index `at`; equal sort keys keep their
+// order).  src[o]: the caller's position of internal element o.  below[i]
+// (split only): elements of row i below `at`.  Rows are processed in
+// parallel, in chunks taken from a shared counter.
 void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
               const std::vector<uint32_t>& perm, const std::vector<uint32_t>& ipos, std::vector<uint32_t>& optr,
-              std::vector<uint32_t>& oidx, std::vector<float>& oval, std::vector<uint32_t>& src, bool sort) {
+              std::vector<uint32_t>& oidx, std::vector<float>& oval, std::vector<uint32_t>& src, RowOrder order,
+              uint32_t at, std::vector<uint32_t>& below) {
     optr.assign((size_t)rows + 1, 0u);
     for (int64_t i = 0; i < rows; ++i) optr[i + 1] = optr[i] + (ptr[perm[i] + 1] - ptr[perm[i]]);
     oidx.resize(optr[rows]);
     oval.resize(optr[rows]);
     src.resize(optr[rows]);
+    below.assign(order == RowOrder::split ? (size_t)rows : 0, 0u);
     constexpr int64_t CHUNK = 512;
     std::atomic<int64_t> next{0};
     auto work = [&]() {
@@ -480,12 +546,18 @@ void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const floa
             for (int64_t i = b; i < std::min(rows, b + CHUNK); ++i) {
                 const uint32_t r = perm[i], e0 = ptr[r], len = ptr[r + 1] - ptr[r];
                 key.resize(len);
-                for (uint32_t q = 0; q < len; ++q) key[q] = (uint64_t)ipos[idx[e0 + q]] << 32 | q;
-                if (sort) std::sort(key.begin(), key.end());
+                uint32_t nb = 0;
+                for (uint32_t q = 0; q < len; ++q) {
+                    const uint32_t k = ipos[idx[e0 + q]];
+                    nb += k < at ? 1u : 0u;
+                    key[q] = (uint64_t)(order == RowOrder::sort ? k : (k >= at ? 1u : 0u)) << 32 | q;
+                }
+                if (order != RowOrder::keep) std::sort(key.begin(), key.end());
+                if (order == RowOrder::split) below[i] = nb;
                 const uint32_t o = optr[i];
                 for (uint32_t q = 0; q < len; ++q) {
                     const uint32_t e = e0 + (uint32_t)key[q];
-                    oidx[o + q] = (uint32_t)(key[q] >> 32);
+                    oidx[o + q] = ipos[idx[e]];
                     oval[o + q] = val[e];
                     src[o + q] = e;
                 }
@@ -605,9 +677,9 @@ int32_t ccd_set_plan(ccd_ctx* c, const ccd_plan_params* pp) {
     if (!c || !pp) return CCD_ERR_INVALID;
     if (pp->elems_per_lane == 0 || pp->piece == 0 || pp->unit_elems == 0 || pp->split_threshold < pp->piece)
         return set_err(c, CCD_ERR_INVALID, "plan: elems_per_lane, piece, unit_elems > 0 and split_threshold >= piece");
-    if (pp->sigma != 8 && pp->sigma != 16 && pp->sigma != 32)
-        return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8, 16 or 32");
+    if (pp->sigma != 8 && pp->sigma != 16) return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8 or 16");
     if (pp->blocks_per_cu > 2) return set_err(c, CCD_ERR_INVALID, "plan: blocks_per_cu must be 0, 1 or 2");
+    if (pp->split_rows > 1) return set_err(c, CCD_ERR_INVALID, "plan: split_rows must be 0 or 1");
     c->plan = *pp;
     ++c->gen;
     return CCD_OK;
@@ -640,18 +712,27 @@ int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_
         for (int64_t j = 0; j < n; ++j) cpos[cperm[j]] = (uint32_t)j;
         std::vector<uint32_t> p2, i2, src;
         std::vector<float> v2;
-        // Rows are sorted by gathered index only when the whole gathered
-        // vector fits the LDS prefix of a plain sweep (FP32 entries of one
-        // block per CU, as launch_shape sizes it).
-        const int64_t lds_entries = std::max(0, std::min(c->lds_block, c->lds_cu) - 1024) / 4;
-        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2, src, n <= lds_entries);
+        // The LDS prefix of a plain sweep (FP32 entries of one block per CU,
+        // as launch_shape sizes it).  A layout whose gathered vector fits it
+        // gets its rows sorted by gathered index; a larger one gets each row
+        // split at it, so the tiled sweeps can keep the elements gathered
+        // from beyond the prefix in tiles of their own (build_tiled).
+        const uint32_t lds_entries =
+            (uint32_t)(((size_t)std::max(0, std::min(c->lds_block, c->lds_cu) - 1024) & ~(size_t)255) / 4);
+        std::vector<uint32_t> below;
+        auto order_for = [&](int64_t gdim) { return gdim <= lds_entries ? RowOrder::sort : RowOrder::split; };
+        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2, src, order_for(n), lds_entries, below);
         int rc = upload_layout(c, c->csr, m, n, nnz, p2.data(), i2.data(), v2.data());
         if (rc) return rc;
         c->csr.src = std::move(src);
-        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2, src, m <= lds_entries);
+        c->csr.below = std::move(below);
+        c->csr.split_at = order_for(n) == RowOrder::split ? lds_entries : 0u;
+        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2, src, order_for(m), lds_entries, below);
         rc = upload_layout(c, c->csc, n, m, nnz, p2.data(), i2.data(), v2.data());
         if (rc) return rc;
         c->csc.src = std::move(src);
+        c->csc.below = std::move(below);
+        c->csc.split_at = order_for(m) == RowOrder::split ? lds_entries : 0u;
         c->csr.perm = std::move(rperm);
         c->csc.perm = std::move(cperm);
     }
@@ -801,13 +882,13 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         c->kind_ms[q] = 0.0;
         c->kind_n[q] = 0;
     }
-    const bool tl = p->variant == CCD_VARIANT_TILED;
-    c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, tl, false, 1, 0, true);
-    c->kind_bytes[CCD_K_SWEEP_S] = layout_bytes(S, tl, false, 1, 0, true);
-    c->kind_bytes[CCD_K_FUSED_F] = layout_bytes(F, tl, true, 2, 2, true);
-    c->kind_bytes[CCD_K_FUSED_S] = layout_bytes(S, tl, true, 3, 2, true);
+    const bool tl = p->variant == CCD_VARIANT_TILED, sp = c->plan.split_rows != 0;
+    c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, tl, sp, false, 1, 0, true);
+    c->kind_bytes[CCD_K_SWEEP_S] = layout_bytes(S, tl, sp, false, 1, 0, true);
+    c->kind_bytes[CCD_K_FUSED_F] = layout_bytes(F, tl, sp, true, 2, 2, true);
+    c->kind_bytes[CCD_K_FUSED_S] = layout_bytes(S, tl, sp, true, 3, 2, true);
     c->kind_bytes[CCD_K_RESIDUAL] =
-        0.5 * (layout_bytes(F, tl, true, 1, 1, false) + layout_bytes(S, tl, true, 1, 1, false));
+        0.5 * (layout_bytes(F, tl, sp, true, 1, 1, false) + layout_bytes(S, tl, sp, true, 1, 1, false));
     c->kind_bytes[CCD_K_FINALIZE] = 8.0 * (F.npieces + S.npieces) / 2.0;
     c->kind_bytes[CCD_K_RMSE] = 12.0 * (double)c->nnz_test + 4.0 * (double)k * (double)(c->m + c->n);
     c->kind_bytes[CCD_K_COPY] = 5.0 * (p->precision == CCD_PREC_BF16 ? 2.0 : 4.0) * (double)(c->m + c->n);
@@ -815,12 +896,14 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     const bool tiled = p->variant == CCD_VARIANT_TILED;
     const PassFn pass = tiled ? do_tiled_pass : do_pass;
     if (tiled) {
-        int rc0 = build_tiled(c, F, (int)c->plan.sigma);
-        if (!rc0) rc0 = build_tiled(c, S, (int)c->plan.sigma);
+        int rc0 = build_tiled(c, F, (int)c->plan.sigma, c->plan.split_rows != 0);
+        if (!rc0) rc0 = build_tiled(c, S, (int)c->plan.sigma, c->plan.split_rows != 0);
         if (rc0) return rc0;
         // residual into tile order; rows without entries keep y = 0
-        if (F.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(F.val, F.tl.tval, F.nnz, F.tl.ntiles, F.tl.sigma, true, c->stream));
-        if (S.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(S.val, S.tl.tval, S.nnz, S.tl.ntiles, S.tl.sigma, true, c->stream));
+        if (F.tl.ntiles)
+            HIP_TRY(c, ccd::launch_retile_f32(F.val, F.tl.tval, F.nnz, F.tl.ntiles, F.tl.sigma, true, c->stream, F.tl.vpos));
+        if (S.tl.ntiles)
+            HIP_TRY(c, ccd::launch_retile_f32(S.val, S.tl.tval, S.nnz, S.tl.ntiles, S.tl.sigma, true, c->stream, S.tl.vpos));
         if (bf) {
             HIP_TRY(c, ccd::launch_fill_bytes16(reinterpret_cast<uint16_t*>(vF), 0, (uint64_t)dF, c->stream));
             HIP_TRY(c, ccd::launch_fill_bytes16(reinterpret_cast<uint16_t*>(vS), 0, (uint64_t)dS, c->stream));
@@ -986,8 +1069,10 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         HIP_TRY(c, ccd::launch_from_bf16(c->H16, c->H, (uint64_t)k * c->n, c->stream));
     }
     if (tiled) {
-        if (F.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(F.tl.tval, F.val, F.nnz, F.tl.ntiles, F.tl.sigma, false, c->stream));
-        if (S.tl.ntiles) HIP_TRY(c, ccd::launch_retile_f32(S.tl.tval, S.val, S.nnz, S.tl.ntiles, S.tl.sigma, false, c->stream));
+        if (F.tl.ntiles)
+            HIP_TRY(c, ccd::launch_retile_f32(F.tl.tval, F.val, F.nnz, F.tl.ntiles, F.tl.sigma, false, c->stream, F.tl.vpos));
+        if (S.tl.ntiles)
+            HIP_TRY(c, ccd::launch_retile_f32(S.tl.tval, S.val, S.nnz, S.tl.ntiles, S.tl.sigma, false, c->stream, S.tl.vpos));
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (T > 0) c->continued = true;

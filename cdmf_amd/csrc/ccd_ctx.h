@@ -32,6 +32,17 @@ struct TiledLayout {
     ccd::SpanRow* spans = nullptr;  // rows spanning tiles: two-tile ones first
     uint32_t nspan = 0;
     uint32_t nshort = 0;
+    // Split layout (rows split at DevLayout::split_at): the tiles hold
+    // virtual rows.  Virtual row r < nreal is the part of row r gathered
+    // from below the split, virtual row nreal + k the rest of row
+    // cold_row[k].  Sweeps leave (g, h) per virtual row in part; the combine
+    // kernel adds the two parts of each row and finishes it.
+    bool split = false;
+    uint32_t nreal = 0;
+    uint32_t* vpos = nullptr;      // position in the layout of each element, in tiled order
+    uint32_t* cold_row = nullptr;  // real row of each second part
+    uint2* comb = nullptr;         // per row: {virtual row of its second part or 0xffffffff, first part's length}
+    double2* part = nullptr;       // per virtual row, double-sized so every precision fits
 };
 
 struct DevLayout {
@@ -62,6 +73,10 @@ struct DevLayout {
     uint32_t split_x = 0, split_p = 0;
     std::vector<uint32_t> hptr;      // host copy of the internal ptr (planning)
     std::vector<uint32_t> src;       // caller's position of each internal element (residual download)
+    // Rows split at internal index split_at (0: rows sorted, not split):
+    // below[i] leading elements of row i are gathered from below it.
+    uint32_t split_at = 0;
+    std::vector<uint32_t> below;
     std::vector<uint32_t> perm;      // internal row -> caller's row
     TiledLayout tl;
 };
@@ -109,7 +124,7 @@ struct ccd_ctx {
     double* obj_dev = nullptr;   // objective, loss, regulariser
     int32_t rmse_cap = 0;
     std::vector<double> iter_rmse;
-    ccd_plan_params plan{4, 8192, 4096, 4096, 16, 0};
+    ccd_plan_params plan{4, 8192, 4096, 4096, 16, 0, 1};
     // profiling
     bool prof = false;
     uint32_t prof_kinds = 0;  // bitmask of timed kernel kinds (1 << ccd_kernel_kind)
@@ -149,6 +164,6 @@ void drop_graph(ccd_ctx* c);
 // up to 2^24, else 4.
 int tiled_index_bytes(const DevLayout& L);
 // Tile descriptors of layout L for tiles of 64 * sigma elements (cached).
-int build_tiled(ccd_ctx* c, DevLayout& L, int sigma);
+int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split);
 
 }  // namespace ccdi

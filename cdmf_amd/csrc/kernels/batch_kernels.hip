@@ -102,6 +102,9 @@ __device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const Pas
     a.segrows = L.segrows;
     a.tile_open = L.tile_open;
     a.tile_gmax = nullptr;  // one tile_core copy here: every tile takes the general path
+    a.cold_row = nullptr;   // the batched kernel works on plain layouts
+    a.nreal = 0;
+    a.part = nullptr;
     a.head = L.head;
     a.tail = L.tail;
     a.nnz = L.nnz;

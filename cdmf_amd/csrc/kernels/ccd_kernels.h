@@ -296,6 +296,13 @@ struct TiledArgs {
     uint32_t hot_p;
     uint32_t x_is_cur;
     float lambda;
+    // Split layout (tiles of virtual rows, see build_tiled): a sweep leaves
+    // (g, h) of every virtual row in part instead of finishing rows, and the
+    // per-row vectors of virtual row v >= nreal are those of row
+    // cold_row[v - nreal].  All null / 0 for a plain layout.
+    const uint32_t* cold_row;
+    uint32_t nreal;
+    void* part;  // (g, h) pairs of the accumulator type
 };
 
 // One row spanning tiles t0..t1 (t1 > t0) of a tiled layout: its row, first
@@ -314,7 +321,22 @@ struct SpanArgs {
     uint32_t nspan;
     uint32_t nshort;
     float lambda;
+    void* part;  // split layout: (g, h) of each spanning virtual row go here instead of finishing it
 };
+
+// Split layout: y[r] = (g1 + g2) / (lambda |row r| + (h1 + h2)) from the
+// (g, h) of the row's two virtual rows (first part r, second part
+// comb[r].x unless 0xffffffff; a first part of length comb[r].y = 0 is left
+// out); rows without entries keep y.
+struct CombineArgs {
+    const void* part;
+    const uint2* comb;
+    const uint32_t* ptr;
+    void* y;
+    float lambda;
+    uint32_t nreal;
+};
+hipError_t launch_combine(const CombineArgs& a, int prec, hipStream_t s);
 
 // Index formats of the tiled copies, by bytes per index: 2 (u16), 3 (a u16
 // plane plus a u8 plane of bits 16..23, for gathered lengths up to 2^24) or
@@ -323,22 +345,26 @@ struct SpanArgs {
 // bytes of lane l of tile t are the sigma bytes at t * 64 * sigma + l * sigma.
 struct Idx24 {};
 
-// sigma 8, 16 or 32 for FP32; 16 for the other precisions; ib: bytes per index
+// sigma 8 or 16 for FP32, 16 for the other precisions (at sigma 32 the
+// kernels spill hundreds of bytes per lane, run slower, and the fused
+// passes' compiled code gave wrong results); ib: bytes per index
 hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, int ib, bool sub, bool add, bool sweep,
                         uint32_t blocks, hipStream_t s, const LaunchEvents* ev = nullptr);
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s);
 int tiled_blocks_per_cu(int sigma, int prec, int ib, bool sub, bool add, bool sweep, bool x_is_cur);
-// original order <-> tiled order (to_tiled = true: src original, dst tiled)
+// Layout order <-> tiled order (to_tiled = true: src in layout order, dst
+// tiled).  map (optional, a split layout's vpos): layout position of each
+// element in tiled order; without it the two orders are the same.
 hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_t ntiles, int sigma, bool to_tiled,
-                             hipStream_t s);
+                             hipStream_t s, const uint32_t* map = nullptr);
 // u16 or u32 indices in, the tiled copy in the format of ib out
 hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uint64_t nnz, uint32_t ntiles, int sigma,
-                             hipStream_t s);
-// gmax[t] = largest of the indices of tile t (original order), 0xffffffff
-// for a tile that extends past nnz or holds empty rows (segoff[t] other
-// than 0xffffffff)
+                             const uint32_t* map, hipStream_t s);
+// gmax[t] = largest of the indices of tile t (tiled order), 0xffffffff for a
+// tile that extends past nnz or holds empty rows (segoff[t] other than
+// 0xffffffff)
 hipError_t launch_tile_gmax(const void* idx, bool idx16, const uint32_t* segoff, uint32_t* gmax, uint64_t nnz,
-                            uint32_t ntiles, int sigma, hipStream_t s);
+                            uint32_t ntiles, int sigma, const uint32_t* map, hipStream_t s);
 hipError_t launch_fill(float* dst, float value, uint64_t count, hipStream_t s);
 hipError_t launch_fill_bytes16(uint16_t* dst, uint16_t value, uint64_t count, hipStream_t s);
 

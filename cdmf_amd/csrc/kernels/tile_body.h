@@ -30,10 +30,17 @@ struct Pair {
 };
 
 // y_row = g / (lambda |row| + h); the row's length d comes from the tile's
-// row-start flags, not from two row-pointer loads per finished row.
+// row-start flags, not from two row-pointer loads per finished row.  In a
+// split layout the virtual row's (g, h) is kept for the combine kernel.
 template <typename TF, typename TA>
 __device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA g, TA h, uint32_t d) {
-    stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));
+    if (a.part) static_cast<Pair<TA>*>(a.part)[row] = Pair<TA>{g, h};
+    else stf<TF>(a.y, row, (float)(g / ((TA)a.lambda * (TA)d + h)));
+}
+
+// Row whose per-row vectors a (virtual) row of the tiles uses.
+__device__ __forceinline__ uint32_t real_row(const TiledArgs& a, uint32_t v) {
+    return a.cold_row && v >= a.nreal ? a.cold_row[v - a.nreal] : v;
 }
 
 // Index plane element: u16 (also the low plane of the 3-byte format) or
@@ -128,18 +135,19 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
 // One tile after its streams are loaded: the lane's per-row vectors and
 // gathers, the residual update, per-row sums and row finishes.  SEG: how the
 // tile's rows are found (seg_row).  XCUR (fused passes only): the sweep's x
-// is the pair's current value.  HOT (fused passes only): every gather of the
-// tile is below the staged counts and the tile has no padding (it is not a
+// is the pair's current value.  HOT (sweeps only): every gather of the tile
+// is below the staged counts and the tile has no padding (it is not a
 // partial last tile), so the gathers are plain LDS reads and no element is
-// masked.
+// masked.  FLAT (plain sweeps of the batched kernel): one flat load per
+// gather instead of separate LDS and global loads.
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR, int SEG,
-          bool HOT = false>
+          bool HOT = false, bool FLAT = false>
 __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
                                           const int lane, const uint32_t flags, const uint32_t seg0,
                                           const uint32_t row0, const uint32_t segoff, const uint32_t open,
                                           const uint32_t (&c)[SIGMA], float (&r)[SIGMA]) {
     constexpr bool PAIR = SWEEP && (SUB || ADD);
-    static_assert(PAIR || !HOT, "only fused passes have an LDS-only path");
+    static_assert(SWEEP || !HOT, "only sweeps gather from LDS");
     constexpr uint32_t TS = WAVE * SIGMA;
     float* __restrict__ tval = a.tval + (size_t)tile * TS;
     const uint64_t lbase = (uint64_t)tile * TS + (uint64_t)lane * SIGMA;
@@ -148,27 +156,24 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
     // Per-row vectors of the lane's first two rows (a lane of SIGMA elements
     // rarely holds more), loaded before the gathers so that their latency
     // overlaps; a third row's are loaded where it starts.
-    float ysr = SUB ? ldf<TF>(a.ys, row) : 0.0f;
-    float yar = ADD ? ldf<TF>(a.ya, row) : 0.0f;
+    float ysr = SUB ? ldf<TF>(a.ys, real_row(a, row)) : 0.0f;
+    float yar = ADD ? ldf<TF>(a.ya, real_row(a, row)) : 0.0f;
     float ysn = 0.0f, yan = 0.0f;
     if constexpr (SUB || ADD) {
         if (flags >> 1) {
             const uint32_t rn = seg_row<SEG>(seg + 1, row0, segoff, a.segrows);
-            if (SUB) ysn = ldf<TF>(a.ys, rn);
-            if (ADD) yan = ldf<TF>(a.ya, rn);
+            if (SUB) ysn = ldf<TF>(a.ys, real_row(a, rn));
+            if (ADD) yan = ldf<TF>(a.ya, real_row(a, rn));
         }
     }
 
     // Every gather of the lane is issued before the tile's first store, so
-    // all of them are in flight together rather than one per element.
-    // Fused passes: HOT tiles read LDS only; other tiles read LDS for every
-    // element (entry 0 past the staged prefix), then overwrite the elements
-    // past it with global loads.  All LDS reads are issued first, so the
-    // global loads wait for them once rather than one by one (a global load
-    // into a register must wait for the LDS read it overwrites).  Plain
-    // sweeps gather with one flat load per element (gather): with the
-    // unsorted rows of a gathered vector larger than the LDS prefix, this
-    // measured faster than separate LDS and global loads.
+    // all of them are in flight together rather than one per element.  HOT
+    // tiles read LDS only.  Other tiles read LDS for every element (entry 0
+    // past the staged prefix), then overwrite the elements past it with
+    // global loads.  All LDS reads are issued first, so the global loads
+    // wait for them once rather than one by one (a global load into a
+    // register must wait for the LDS read it overwrites).
     float xsv[SIGMA], xav[SIGMA], xv[SIGMA];
     const lds_ptr<TF> l3 = (lds_ptr<TF>)lds;
     const lds_ptr<PairWord<TF>> p3 = (lds_ptr<PairWord<TF>>)ldsp;
@@ -195,12 +200,28 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
             xav[i] = tof<TF>(p.cur);
             xv[i] = XCUR ? xav[i] : tof<TF>(xg[i]);
         }
-    } else if constexpr (SWEEP) {
+    } else if constexpr (SWEEP && FLAT) {
 #pragma unroll
         for (int i = 0; i < SIGMA; ++i) {
             xsv[i] = 0.0f;
             xav[i] = 0.0f;
             xv[i] = gather<TF>(lds, a.hot_x, a.x, c[i]);
+        }
+    } else if constexpr (SWEEP) {
+        TF xg[SIGMA];
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) xg[i] = l3[HOT || c[i] < a.hot_x ? c[i] : 0u];
+        if constexpr (!HOT) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < SIGMA; ++i)
+                if (c[i] >= a.hot_x) xg[i] = ((glb_ptr<TF>)a.x)[c[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) {
+            xsv[i] = 0.0f;
+            xav[i] = 0.0f;
+            xv[i] = tof<TF>(xg[i]);
         }
     } else {
 #pragma unroll
@@ -237,8 +258,8 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
                 ysr = ysn;
                 yar = yan;
             } else {
-                if (SUB) ysr = ldf<TF>(a.ys, row);
-                if (ADD) yar = ldf<TF>(a.ya, row);
+                if (SUB) ysr = ldf<TF>(a.ys, real_row(a, row));
+                if (ADD) yar = ldf<TF>(a.ya, real_row(a, row));
             }
         }
         const bool valid = HOT || lbase + (uint64_t)i < a.nnz;
@@ -315,28 +336,29 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
     }
 }
 
-// Largest gathered index for which a tile of a fused pass is HOT: the
-// staged count of everything the pass gathers (the pairs, and x unless it
-// is the pair's current value), at most 2^16 for 3-byte indices (HOT tiles
-// skip the high bytes).
-template <typename IdxT, bool XCUR>
+// Largest gathered index for which a tile of a sweep is HOT: the staged
+// count of everything the sweep gathers (x; for a fused pass the pairs, and
+// x unless it is the pair's current value), at most 2^16 for 3-byte indices
+// (HOT tiles skip the high bytes).
+template <typename IdxT, bool SUB, bool ADD, bool XCUR>
 __device__ __forceinline__ uint32_t hot_limit(const TiledArgs& a) {
     uint32_t lim;
-    if constexpr (XCUR) lim = a.hot_p;
+    if constexpr (!(SUB || ADD)) lim = a.hot_x;
+    else if constexpr (XCUR) lim = a.hot_p;
     else lim = a.hot_p < a.hot_x ? a.hot_p : a.hot_x;
     if constexpr (std::is_same<IdxT, Idx24>::value) lim = lim < 0x10000u ? lim : 0x10000u;
     return lim;
 }
 
 // One tile: streams, then tile_core.  `tile` is wave-uniform, so the
-// per-tile descriptors are scalar loads.  SPLIT: separate code for fused-
-// pass tiles whose gathers all hit LDS (common when each row's elements are
-// sorted by gathered index), for the other tiles with consecutive rows and
-// for tiles with empty rows (faster; the batched kernel, which inlines every
-// pass mode, keeps one copy to stay within its registers).  gmax: the tile's
-// tile_gmax entry (fused passes), loaded by the caller one tile ahead so
-// that the choice is made before the tile's streams are issued (each copy
-// issues its own).
+// per-tile descriptors are scalar loads.  SPLIT: separate code for sweep
+// tiles whose gathers all hit LDS (most tiles once rows are sorted or split
+// by gathered index), for the other tiles with consecutive rows and for
+// tiles with empty rows (faster; the batched kernel, which inlines every
+// pass mode, keeps one copy, with flat gathers, to stay within its
+// registers).  gmax: the tile's tile_gmax entry (sweeps), loaded by the
+// caller one tile ahead so that the choice is made before the tile's
+// streams are issued (each copy issues its own).
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR,
           bool SPLIT = true>
 __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
@@ -358,27 +380,27 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
     float r[SIGMA];
     uint32_t flags, seg0;
 
-    // A fused-pass tile with gmax below the staged counts has consecutive
-    // rows (tiles with empty rows carry 0xffffffff) and gathers only from
-    // LDS.  The other tiles' rows are consecutive unless the tile holds empty
+    // A sweep tile with gmax below the staged counts has consecutive rows
+    // (tiles with empty rows carry 0xffffffff) and gathers only from LDS.
+    // The other tiles' rows are consecutive unless the tile holds empty
     // rows.  Copies of the rest are chosen by wave-uniform branches, so the
     // common one has no segment-table loads (a load on only one of two merged
     // paths makes the compiler wait for every outstanding load at each later
     // use), no global gathers and, for 3-byte indices, no high bytes.
-    constexpr bool PAIR = SWEEP && (SUB || ADD);
-    if (SPLIT && PAIR && gmax < hot_limit<IdxT, XCUR>(a)) {
-        descriptors();
-        load_tile<IdxT, SIGMA, true, false>(a, tile, lane, c, r, flags, seg0);
-        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN, PAIR>(a, lds, ldsp, tile, lane, flags, seg0,
-                                                                           row0, segoff, open, c, r);
+    if (SPLIT && SWEEP && gmax < hot_limit<IdxT, SUB, ADD, XCUR>(a)) {
+        if constexpr (EARLY) descriptors();
+        load_tile<IdxT, SIGMA, EARLY, false>(a, tile, lane, c, r, flags, seg0);
+        if constexpr (!EARLY) descriptors();
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN, SWEEP>(a, lds, ldsp, tile, lane, flags, seg0,
+                                                                            row0, segoff, open, c, r);
         return;
     }
     if constexpr (EARLY) descriptors();
     load_tile<IdxT, SIGMA, EARLY>(a, tile, lane, c, r, flags, seg0);
     if constexpr (!EARLY) descriptors();
     if constexpr (!SPLIT)
-        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_ANY>(a, lds, ldsp, tile, lane, flags, seg0, row0,
-                                                                     segoff, open, c, r);
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_ANY, false, true>(a, lds, ldsp, tile, lane, flags,
+                                                                                  seg0, row0, segoff, open, c, r);
     else if (segoff == 0xffffffffu)
         tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN>(a, lds, ldsp, tile, lane, flags, seg0, row0,
                                                                      segoff, open, c, r);
@@ -397,7 +419,8 @@ __device__ __forceinline__ void span_short(const SpanArgs& a, uint32_t i) {
     const Pair<TA> hd = static_cast<const Pair<TA>*>(a.head)[sp.t1];
     const TA g = tl.g + ((TA)0 + hd.g);
     const TA h = tl.h + ((TA)0 + hd.h);
-    stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
+    if (a.part) static_cast<Pair<TA>*>(a.part)[sp.row] = Pair<TA>{g, h};
+    else stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
 }
 
 // Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], by one
@@ -424,8 +447,32 @@ __device__ __forceinline__ void span_row(const SpanArgs& a, uint32_t i, int lane
     if (lane == 0) {
         g = tl.g + g;
         h = tl.h + h;
-        stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
+        if (a.part) static_cast<Pair<TA>*>(a.part)[sp.row] = Pair<TA>{g, h};
+        else stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
     }
+}
+
+// One row of a split layout: its two virtual rows' (g, h), added in a fixed
+// order (first part, then second), then the row update.
+template <typename TF, typename TA>
+__device__ __forceinline__ void combine_row(const CombineArgs& a, uint32_t r) {
+    const uint32_t len = a.ptr[r + 1] - a.ptr[r];
+    if (len == 0) return;
+    const uint2 cb = a.comb[r];
+    const Pair<TA>* P = static_cast<const Pair<TA>*>(a.part);
+    TA g, h;
+    if (cb.y) {
+        g = P[r].g;
+        h = P[r].h;
+        if (cb.x != 0xffffffffu) {
+            g = g + P[cb.x].g;
+            h = h + P[cb.x].h;
+        }
+    } else {
+        g = P[cb.x].g;
+        h = P[cb.x].h;
+    }
+    stf<TF>(a.y, r, (float)(g / ((TA)a.lambda * (TA)len + h)));
 }
 
 }  // namespace tiled

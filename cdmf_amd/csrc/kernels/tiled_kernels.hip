@@ -49,15 +49,14 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     const uint32_t stride = gridDim.x * NW;
     // wave-uniform tile index: the per-tile descriptors become scalar loads
     const uint32_t first = __builtin_amdgcn_readfirstlane(blockIdx.x * NW + (threadIdx.x >> 6));
-    // Fused passes: each tile's tile_gmax entry is loaded while the previous
-    // tile runs (unconditionally, at a clamped tile index: a load on only one
+    // Sweeps: each tile's tile_gmax entry is loaded while the previous tile
+    // runs (unconditionally, at a clamped tile index: a load on only one
     // of two merged paths would make the compiler drain every outstanding
     // load at the end of each tile).
-    constexpr bool PAIR = SWEEP && (SUB || ADD);
-    uint32_t gmax = PAIR && first < a.ntiles ? ld_const(a.tile_gmax, first) : 0xffffffffu;
+    uint32_t gmax = SWEEP && first < a.ntiles ? ld_const(a.tile_gmax, first) : 0xffffffffu;
     for (uint32_t tile = first; tile < a.ntiles; tile += stride) {
         const uint32_t next = tile + stride < a.ntiles ? tile + stride : tile;
-        const uint32_t gnext = PAIR ? ld_const(a.tile_gmax, next) : 0xffffffffu;
+        const uint32_t gnext = SWEEP ? ld_const(a.tile_gmax, next) : 0xffffffffu;
         tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane, gmax);
         gmax = gnext;
     }
@@ -78,23 +77,25 @@ __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
     tiled::span_row<TF, TA>(a, i, threadIdx.x & (WAVE - 1));
 }
 
+// map (optional): layout position of each element in tiled order
 __global__ __launch_bounds__(BLOCK) void retile_f32_kernel(const float* __restrict__ src, float* __restrict__ dst,
-                                                           uint64_t nnz, uint64_t padded, uint32_t sigma,
-                                                           bool to_tiled) {
+                                                           const uint32_t* __restrict__ map, uint64_t nnz,
+                                                           uint64_t padded, uint32_t sigma, bool to_tiled) {
     const uint64_t ts = (uint64_t)WAVE * sigma;
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
         const uint64_t tile = e / ts, rem = e % ts;
         const uint64_t i = rem % sigma, l = rem / sigma;
         const uint64_t pos = tile * ts + (i >> 2) * (4 * WAVE) + 4 * l + (i & 3);
-        if (to_tiled) dst[pos] = e < nnz ? src[e] : 0.0f;
-        else if (e < nnz) dst[e] = src[pos];
+        if (to_tiled) dst[pos] = e < nnz ? src[map ? map[e] : e] : 0.0f;
+        else if (e < nnz) dst[map ? map[e] : e] = src[pos];
     }
 }
 
 template <typename IdxT>
 __global__ __launch_bounds__(BLOCK) void retile_idx_kernel(const IdxT* __restrict__ src, IdxT* __restrict__ dst,
-                                                           uint64_t nnz, uint64_t padded, uint32_t sigma) {
+                                                           const uint32_t* __restrict__ map, uint64_t nnz,
+                                                           uint64_t padded, uint32_t sigma) {
     const uint64_t ts = (uint64_t)WAVE * sigma;
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(BLOCK) void retile_idx_kernel(const IdxT* __restric
         const uint64_t i = rem % sigma, l = rem / sigma;
         constexpr uint64_t G = 16 / sizeof(IdxT);  // indices per 16-byte lane load
         const uint64_t pos = tile * ts + (i / G) * (G * WAVE) + G * l + (i % G);
-        dst[pos] = e < nnz ? src[e] : (IdxT)0;
+        dst[pos] = e < nnz ? src[map ? map[e] : e] : (IdxT)0;
     }
 }
 
@@ -110,16 +111,23 @@ __global__ __launch_bounds__(BLOCK) void retile_idx_kernel(const IdxT* __restric
 // position of the u16 layout, bits 16..23 in the lane-major u8 plane.
 __global__ __launch_bounds__(BLOCK) void retile_idx24_kernel(const uint32_t* __restrict__ src,
                                                              uint16_t* __restrict__ lo, uint8_t* __restrict__ hi,
-                                                             uint64_t nnz, uint64_t padded, uint32_t sigma) {
+                                                             const uint32_t* __restrict__ map, uint64_t nnz,
+                                                             uint64_t padded, uint32_t sigma) {
     const uint64_t ts = (uint64_t)WAVE * sigma;
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
     for (uint64_t e = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; e < padded; e += stride) {
         const uint64_t tile = e / ts, rem = e % ts;
         const uint64_t i = rem % sigma, l = rem / sigma;
-        const uint32_t v = e < nnz ? src[e] : 0u;
+        const uint32_t v = e < nnz ? src[map ? map[e] : e] : 0u;
         lo[tile * ts + (i >> 3) * (8 * WAVE) + 8 * l + (i & 7)] = (uint16_t)(v & 0xffffu);
         hi[e] = (uint8_t)(v >> 16);  // e = tile * ts + l * sigma + i
     }
+}
+
+template <typename TF, typename TA>
+__global__ __launch_bounds__(BLOCK) void combine_kernel(CombineArgs a) {
+    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r < a.nreal) tiled::combine_row<TF, TA>(a, r);
 }
 
 // Largest index of each tile, one wave per tile (grid-stride over tiles);
@@ -128,6 +136,7 @@ __global__ __launch_bounds__(BLOCK) void retile_idx24_kernel(const uint32_t* __r
 template <typename IdxT>
 __global__ __launch_bounds__(BLOCK) void tile_gmax_kernel(const IdxT* __restrict__ idx,
                                                           const uint32_t* __restrict__ segoff,
+                                                          const uint32_t* __restrict__ map,
                                                           uint32_t* __restrict__ gmax, uint64_t nnz, uint32_t ntiles,
                                                           uint32_t sigma) {
     const uint64_t ts = (uint64_t)WAVE * sigma;
@@ -136,7 +145,7 @@ __global__ __launch_bounds__(BLOCK) void tile_gmax_kernel(const IdxT* __restrict
         const uint64_t base = (uint64_t)t * ts;
         uint32_t mx = 0;
         for (uint64_t e = base + lane; e < base + ts && e < nnz; e += WAVE) {
-            const uint32_t v = idx[e];
+            const uint32_t v = idx[map ? map[e] : e];
             mx = v > mx ? v : mx;
         }
 #pragma unroll
@@ -191,7 +200,6 @@ TiledFn tiled_fn_i(int sigma, int prec, int code) {
         switch (sigma) {
             case 8: return tiled_fn_s<IdxT, 8, float, float>(code);
             case 16: return tiled_fn_s<IdxT, 16, float, float>(code);
-            case 32: return tiled_fn_s<IdxT, 32, float, float>(code);
             default: return nullptr;
         }
     }
@@ -251,29 +259,41 @@ hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_combine(const CombineArgs& a, int prec, hipStream_t s) {
+    if (a.nreal == 0) return hipSuccess;
+    const dim3 grid((a.nreal + BLOCK - 1) / BLOCK);
+    switch (prec) {
+        case PREC_F32: hipLaunchKernelGGL((combine_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_BF16: hipLaunchKernelGGL((combine_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_F64SUM: hipLaunchKernelGGL((combine_kernel<float, double>), grid, dim3(BLOCK), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_retile_f32(const float* src, float* dst, uint64_t nnz, uint32_t ntiles, int sigma, bool to_tiled,
-                             hipStream_t s) {
+                             hipStream_t s, const uint32_t* map) {
     const uint64_t padded = (uint64_t)ntiles * WAVE * (uint64_t)sigma;
-    hipLaunchKernelGGL(retile_f32_kernel, dim3(grid_for(padded)), dim3(BLOCK), 0, s, src, dst, nnz, padded,
+    hipLaunchKernelGGL(retile_f32_kernel, dim3(grid_for(padded)), dim3(BLOCK), 0, s, src, dst, map, nnz, padded,
                        (uint32_t)sigma, to_tiled);
     return hipGetLastError();
 }
 
 hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uint64_t nnz, uint32_t ntiles, int sigma,
-                             hipStream_t s) {
+                             const uint32_t* map, hipStream_t s) {
     const uint64_t padded = (uint64_t)ntiles * WAVE * (uint64_t)sigma;
     const dim3 grid(grid_for(padded));
     if (src16 != (ib == 2)) return hipErrorInvalidValue;
     if (ib == 2) {
         hipLaunchKernelGGL(retile_idx_kernel<uint16_t>, grid, dim3(BLOCK), 0, s, static_cast<const uint16_t*>(src),
-                           static_cast<uint16_t*>(dst), nnz, padded, (uint32_t)sigma);
+                           static_cast<uint16_t*>(dst), map, nnz, padded, (uint32_t)sigma);
     } else if (ib == 3) {
         uint16_t* lo = static_cast<uint16_t*>(dst);
         hipLaunchKernelGGL(retile_idx24_kernel, grid, dim3(BLOCK), 0, s, static_cast<const uint32_t*>(src), lo,
-                           reinterpret_cast<uint8_t*>(lo + padded), nnz, padded, (uint32_t)sigma);
+                           reinterpret_cast<uint8_t*>(lo + padded), map, nnz, padded, (uint32_t)sigma);
     } else if (ib == 4) {
         hipLaunchKernelGGL(retile_idx_kernel<uint32_t>, grid, dim3(BLOCK), 0, s, static_cast<const uint32_t*>(src),
-                           static_cast<uint32_t*>(dst), nnz, padded, (uint32_t)sigma);
+                           static_cast<uint32_t*>(dst), map, nnz, padded, (uint32_t)sigma);
     } else {
         return hipErrorInvalidValue;
     }
@@ -281,16 +301,16 @@ hipError_t launch_retile_idx(const void* src, bool src16, void* dst, int ib, uin
 }
 
 hipError_t launch_tile_gmax(const void* idx, bool idx16, const uint32_t* segoff, uint32_t* gmax, uint64_t nnz,
-                            uint32_t ntiles, int sigma, hipStream_t s) {
+                            uint32_t ntiles, int sigma, const uint32_t* map, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
     uint32_t blocks = (ntiles + WPB - 1) / WPB;
     if (blocks > 16384) blocks = 16384;
     if (idx16)
         hipLaunchKernelGGL(tile_gmax_kernel<uint16_t>, dim3(blocks), dim3(BLOCK), 0, s,
-                           static_cast<const uint16_t*>(idx), segoff, gmax, nnz, ntiles, (uint32_t)sigma);
+                           static_cast<const uint16_t*>(idx), segoff, map, gmax, nnz, ntiles, (uint32_t)sigma);
     else
         hipLaunchKernelGGL(tile_gmax_kernel<uint32_t>, dim3(blocks), dim3(BLOCK), 0, s,
-                           static_cast<const uint32_t*>(idx), segoff, gmax, nnz, ntiles, (uint32_t)sigma);
+                           static_cast<const uint32_t*>(idx), segoff, map, gmax, nnz, ntiles, (uint32_t)sigma);
     return hipGetLastError();
 }
 

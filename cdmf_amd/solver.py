@@ -40,7 +40,8 @@ class CParams(C.Structure):
 
 class CPlan(C.Structure):
     _fields_ = [("elems_per_lane", C.c_uint32), ("split_threshold", C.c_uint32), ("piece", C.c_uint32),
-                ("unit_elems", C.c_uint32), ("sigma", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+                ("unit_elems", C.c_uint32), ("sigma", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("split_rows", C.c_uint32)]
 
 
 class CBatchPlan(C.Structure):
@@ -192,8 +193,8 @@ class Solver:
             pass
 
     def set_plan(self, elems_per_lane=4, split_threshold=8192, piece=4096, unit_elems=4096, sigma=16,
-                 blocks_per_cu=0) -> None:
-        pp = CPlan(elems_per_lane, split_threshold, piece, unit_elems, sigma, blocks_per_cu)
+                 blocks_per_cu=0, split_rows=1) -> None:
+        pp = CPlan(elems_per_lane, split_threshold, piece, unit_elems, sigma, blocks_per_cu, split_rows)
         self._check(self._lib.ccd_set_plan(self._ctx, C.byref(pp)))
 
     def upload(self, ds: Dataset, with_test: bool = True) -> None:

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CCD_ABI_VERSION 3
+#define CCD_ABI_VERSION 4
 
 /* Status codes: 0 = success, negative = error (message: ccd_last_error). */
 #define CCD_OK 0
@@ -189,9 +189,12 @@ typedef struct ccd_plan_params {
     uint32_t split_threshold; /* rows longer than this are split across waves (default 8192) */
     uint32_t piece;           /* elements per piece of a split row (default 4096) */
     uint32_t unit_elems;      /* target elements per wave work unit (default 4096) */
-    uint32_t sigma;           /* elements per lane of a tile of the tiled sweeps: 8, 16 or 32 (default 16) */
+    uint32_t sigma;           /* elements per lane of a tile of the tiled sweeps: 8 or 16 (default 16) */
     uint32_t blocks_per_cu;   /* persistent sweep blocks per CU: 0 = as many as registers allow (1 or 2),
                                  1 or 2 = at most that many; the staged LDS prefix is sized to match */
+    uint32_t split_rows;      /* tiled sweeps, layouts whose gathered vector exceeds the LDS prefix: 1 = keep
+                                 each row's elements gathered from beyond the prefix in tiles of their own,
+                                 finished by a combine pass (default); 0 = one tile range in row order */
 } ccd_plan_params;
 
 int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);

@@ -55,8 +55,9 @@ def _cases():
 
 
 def _single(S, ds, k, lam, outer, inner, order, W0, H0, sigma, runs=1):
+    # the batched kernel runs on plain tile layouts: the single run must too
     with S.Solver(0) as s:
-        s.set_plan(sigma=sigma)
+        s.set_plan(sigma=sigma, split_rows=0)
         s.upload(ds)
         s.set_factors(W0, H0)
         for _ in range(runs):
