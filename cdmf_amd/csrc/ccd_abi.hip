@@ -67,7 +67,7 @@ void free_tiled(TiledLayout& T) {
     dfree(T.spans);
     dfree(T.vpos);
     dfree(T.cold_row);
-    dfree(T.comb);
+    dfree(T.srows);
     dfree(T.part);
     T = TiledLayout{};
 }
@@ -374,20 +374,28 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
     if (ntiles >= 0xffffffffull / 64) return set_err(c, CCD_ERR_INVALID, "too many tiles");
     // virtual rows of a split layout: row pointer, the layout position of
     // every element in tiled order, the real row of every second part and
-    // each row's {second part, first part's length}
+    // each row's parts and their tile ranges
     std::vector<uint32_t> vptr, vpos, cold_row;
-    std::vector<uint2> comb;
+    std::vector<ccd::SplitRow> srows;
     if (split) {
         const int64_t nr = L.rows;
         const uint32_t* hp = L.hptr.data();
         vptr.assign(1, 0u);
         vptr.reserve((size_t)nr * 2 + 1);
-        comb.resize((size_t)nr);
+        srows.resize((size_t)nr);
         for (int64_t r = 0; r < nr; ++r) vptr.push_back(vptr.back() + L.below[r]);
         for (int64_t r = 0; r < nr; ++r) {
             const uint32_t rest = hp[r + 1] - hp[r] - L.below[r];
-            comb[r] = make_uint2(rest ? (uint32_t)(nr + cold_row.size()) : 0xffffffffu, L.below[r]);
+            ccd::SplitRow& q = srows[r];
+            q = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, hp[r + 1] - hp[r]};
+            if (L.below[r]) {
+                q.a0 = (uint32_t)(vptr[r] / TS);
+                q.a1 = (uint32_t)((vptr[r + 1] - 1ull) / TS);
+            }
             if (rest) {
+                q.b = (uint32_t)(nr + cold_row.size());
+                q.b0 = (uint32_t)(vptr.back() / TS);
+                q.b1 = (uint32_t)((vptr.back() + rest - 1ull) / TS);
                 cold_row.push_back((uint32_t)r);
                 vptr.push_back(vptr.back() + rest);
             }
@@ -458,7 +466,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
     if (split) {
         T.nreal = (uint32_t)L.rows;
         if ((rc = upload_vec(c, &T.vpos, vpos)) || (rc = upload_vec(c, &T.cold_row, cold_row)) ||
-            (rc = upload_vec(c, &T.comb, comb)))
+            (rc = upload_vec(c, &T.srows, srows)))
             return rc;
         HIP_TRY(c, dalloc(&T.part, (size_t)rows));
     }
@@ -495,15 +503,14 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
         Timed tm(c, kind, true);
         HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, T.ib, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
     }
-    if (sweep && T.nspan) {
-        Timed tm(c, CCD_K_FINALIZE);
-        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, T.nshort, lambda, T.part};
-        HIP_TRY(c, ccd::launch_span(sa, c->prec, c->stream));
-    }
     if (sweep && T.split) {
         Timed tm(c, CCD_K_FINALIZE);
-        const ccd::CombineArgs ca{T.part, T.comb, L.ptr, y, lambda, T.nreal};
-        HIP_TRY(c, ccd::launch_combine(ca, c->prec, c->stream));
+        const ccd::SplitArgs fa{T.srows, T.part, T.head, T.tail, y, lambda, T.nreal};
+        HIP_TRY(c, ccd::launch_split_finish(fa, c->prec, c->stream));
+    } else if (sweep && T.nspan) {
+        Timed tm(c, CCD_K_FINALIZE);
+        ccd::SpanArgs sa{T.spans, T.head, T.tail, y, T.nspan, T.nshort, lambda, nullptr};
+        HIP_TRY(c, ccd::launch_span(sa, c->prec, c->stream));
     }
     return CCD_OK;
 }

@@ -35,13 +35,14 @@ struct TiledLayout {
     // Split layout (rows split at DevLayout::split_at): the tiles hold
     // virtual rows.  Virtual row r < nreal is the part of row r gathered
     // from below the split, virtual row nreal + k the rest of row
-    // cold_row[k].  Sweeps leave (g, h) per virtual row in part; the combine
-    // kernel adds the two parts of each row and finishes it.
+    // cold_row[k].  Sweeps leave (g, h) per virtual row in part (or as tile
+    // partials); split_finish_kernel adds the two parts of each row and
+    // finishes it.
     bool split = false;
     uint32_t nreal = 0;
     uint32_t* vpos = nullptr;      // position in the layout of each element, in tiled order
     uint32_t* cold_row = nullptr;  // real row of each second part
-    uint2* comb = nullptr;         // per row: {virtual row of its second part or 0xffffffff, first part's length}
+    ccd::SplitRow* srows = nullptr;  // per row: tile ranges of its two parts
     double2* part = nullptr;       // per virtual row, double-sized so every precision fits
 };
 

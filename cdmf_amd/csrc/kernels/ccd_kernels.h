@@ -324,19 +324,28 @@ struct SpanArgs {
     void* part;  // split layout: (g, h) of each spanning virtual row go here instead of finishing it
 };
 
-// Split layout: y[r] = (g1 + g2) / (lambda |row r| + (h1 + h2)) from the
-// (g, h) of the row's two virtual rows (first part r, second part
-// comb[r].x unless 0xffffffff; a first part of length comb[r].y = 0 is left
-// out); rows without entries keep y.
-struct CombineArgs {
+// Row r of a split layout: its first part (virtual row r) over tiles a0..a1
+// (a0 = 0xffffffff: empty), its second part (virtual row b, 0xffffffff:
+// none) over tiles b0..b1, and its length.
+struct SplitRow {
+    uint32_t a0, a1, b, b0, b1, len;
+};
+
+// Finish of a split layout's sweep, one wave per row: each part's (g, h) is
+// the tile kernel's part[] entry when the part lies in one tile, else its
+// tail + heads summed as tiled_span_kernel sums them; then
+// y[r] = (g1 + g2) / (lambda |row r| + (h1 + h2)), the parts added in a
+// fixed order.  Rows without entries keep y.
+struct SplitArgs {
+    const SplitRow* rows;
     const void* part;
-    const uint2* comb;
-    const uint32_t* ptr;
+    const void* head;
+    const void* tail;
     void* y;
     float lambda;
     uint32_t nreal;
 };
-hipError_t launch_combine(const CombineArgs& a, int prec, hipStream_t s);
+hipError_t launch_split_finish(const SplitArgs& a, int prec, hipStream_t s);
 
 // Index formats of the tiled copies, by bytes per index: 2 (u16), 3 (a u16
 // plane plus a u8 plane of bits 16..23, for gathered lengths up to 2^24) or

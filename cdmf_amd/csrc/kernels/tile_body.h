@@ -31,7 +31,7 @@ struct Pair {
 
 // y_row = g / (lambda |row| + h); the row's length d comes from the tile's
 // row-start flags, not from two row-pointer loads per finished row.  In a
-// split layout the virtual row's (g, h) is kept for the combine kernel.
+// split layout the virtual row's (g, h) is kept for split_finish_kernel.
 template <typename TF, typename TA>
 __device__ __forceinline__ void finish_row(const TiledArgs& a, uint32_t row, TA g, TA h, uint32_t d) {
     if (a.part) static_cast<Pair<TA>*>(a.part)[row] = Pair<TA>{g, h};
@@ -423,18 +423,16 @@ __device__ __forceinline__ void span_short(const SpanArgs& a, uint32_t i) {
     else stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
 }
 
-// Rows spanning tiles t0..t1: tail[t0] + head[t0+1] + ... + head[t1], by one
-// wave, lane-strided partial sums reduced in a fixed tree.  The span table
-// holds the tile range and length, so the partials are the only dependent
-// loads.
-template <typename TF, typename TA>
-__device__ __forceinline__ void span_row(const SpanArgs& a, uint32_t i, int lane) {
-    const SpanRow sp = a.spans[i];
-    const Pair<TA>* __restrict__ head = static_cast<const Pair<TA>*>(a.head);
+// tail[t0] + head[t0+1] + ... + head[t1] by one wave: lane-strided partial
+// sums reduced in a fixed tree; the total is valid on lane 0.
+template <typename TA>
+__device__ __forceinline__ Pair<TA> span_sum(const void* headp, const void* tailp, uint32_t t0, uint32_t t1,
+                                             int lane) {
+    const Pair<TA>* __restrict__ head = static_cast<const Pair<TA>*>(headp);
     Pair<TA> tl{0, 0};
-    if (lane == 0) tl = static_cast<const Pair<TA>*>(a.tail)[sp.t0];
+    if (lane == 0) tl = static_cast<const Pair<TA>*>(tailp)[t0];
     TA g = 0, h = 0;
-    for (uint32_t t = sp.t0 + 1 + lane; t <= sp.t1; t += WAVE) {
+    for (uint32_t t = t0 + 1 + lane; t <= t1; t += WAVE) {
         const Pair<TA> p = head[t];
         g += p.g;
         h += p.h;
@@ -444,35 +442,44 @@ __device__ __forceinline__ void span_row(const SpanArgs& a, uint32_t i, int lane
         g += __shfl_xor(g, off);
         h += __shfl_xor(h, off);
     }
+    return Pair<TA>{tl.g + g, tl.h + h};
+}
+
+// Rows spanning tiles t0..t1, by one wave (span_sum).  The span table holds
+// the tile range and length, so the partials are the only dependent loads.
+template <typename TF, typename TA>
+__device__ __forceinline__ void span_row(const SpanArgs& a, uint32_t i, int lane) {
+    const SpanRow sp = a.spans[i];
+    const Pair<TA> s = span_sum<TA>(a.head, a.tail, sp.t0, sp.t1, lane);
     if (lane == 0) {
-        g = tl.g + g;
-        h = tl.h + h;
-        if (a.part) static_cast<Pair<TA>*>(a.part)[sp.row] = Pair<TA>{g, h};
-        else stf<TF>(a.y, sp.row, (float)(g / ((TA)a.lambda * (TA)sp.len + h)));
+        if (a.part) static_cast<Pair<TA>*>(a.part)[sp.row] = s;
+        else stf<TF>(a.y, sp.row, (float)(s.g / ((TA)a.lambda * (TA)sp.len + s.h)));
     }
 }
 
-// One row of a split layout: its two virtual rows' (g, h), added in a fixed
-// order (first part, then second), then the row update.
+// Row r of a split layout, by one wave (SplitArgs).
 template <typename TF, typename TA>
-__device__ __forceinline__ void combine_row(const CombineArgs& a, uint32_t r) {
-    const uint32_t len = a.ptr[r + 1] - a.ptr[r];
-    if (len == 0) return;
-    const uint2 cb = a.comb[r];
+__device__ __forceinline__ void split_row(const SplitArgs& a, uint32_t r, int lane) {
+    const SplitRow sr = a.rows[r];
+    if (sr.len == 0) return;
     const Pair<TA>* P = static_cast<const Pair<TA>*>(a.part);
+    Pair<TA> p1{0, 0}, p2{0, 0};
+    if (sr.a0 != 0xffffffffu) p1 = sr.a0 == sr.a1 ? P[r] : span_sum<TA>(a.head, a.tail, sr.a0, sr.a1, lane);
+    if (sr.b != 0xffffffffu) p2 = sr.b0 == sr.b1 ? P[sr.b] : span_sum<TA>(a.head, a.tail, sr.b0, sr.b1, lane);
+    if (lane != 0) return;
     TA g, h;
-    if (cb.y) {
-        g = P[r].g;
-        h = P[r].h;
-        if (cb.x != 0xffffffffu) {
-            g = g + P[cb.x].g;
-            h = h + P[cb.x].h;
+    if (sr.a0 != 0xffffffffu) {
+        g = p1.g;
+        h = p1.h;
+        if (sr.b != 0xffffffffu) {
+            g = g + p2.g;
+            h = h + p2.h;
         }
     } else {
-        g = P[cb.x].g;
-        h = P[cb.x].h;
+        g = p2.g;
+        h = p2.h;
     }
-    stf<TF>(a.y, r, (float)(g / ((TA)a.lambda * (TA)len + h)));
+    stf<TF>(a.y, r, (float)(g / ((TA)a.lambda * (TA)sr.len + h)));
 }
 
 }  // namespace tiled

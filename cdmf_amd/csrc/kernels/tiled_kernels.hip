@@ -124,10 +124,11 @@ __global__ __launch_bounds__(BLOCK) void retile_idx24_kernel(const uint32_t* __r
     }
 }
 
+// One wave per row of a split layout (tiled::split_row).
 template <typename TF, typename TA>
-__global__ __launch_bounds__(BLOCK) void combine_kernel(CombineArgs a) {
-    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < a.nreal) tiled::combine_row<TF, TA>(a, r);
+__global__ __launch_bounds__(BLOCK) void split_finish_kernel(SplitArgs a) {
+    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (r < a.nreal) tiled::split_row<TF, TA>(a, r, threadIdx.x & (WAVE - 1));
 }
 
 // Largest index of each tile, one wave per tile (grid-stride over tiles);
@@ -259,13 +260,15 @@ hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_combine(const CombineArgs& a, int prec, hipStream_t s) {
+hipError_t launch_split_finish(const SplitArgs& a, int prec, hipStream_t s) {
     if (a.nreal == 0) return hipSuccess;
-    const dim3 grid((a.nreal + BLOCK - 1) / BLOCK);
+    const dim3 grid((a.nreal + WPB - 1) / WPB);
     switch (prec) {
-        case PREC_F32: hipLaunchKernelGGL((combine_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
-        case PREC_BF16: hipLaunchKernelGGL((combine_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
-        case PREC_F64SUM: hipLaunchKernelGGL((combine_kernel<float, double>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_F32: hipLaunchKernelGGL((split_finish_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_BF16: hipLaunchKernelGGL((split_finish_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
+        case PREC_F64SUM:
+            hipLaunchKernelGGL((split_finish_kernel<float, double>), grid, dim3(BLOCK), 0, s, a);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
