@@ -395,6 +395,20 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
                                                                             row0, segoff, open, c, r);
         return;
     }
+    // 3-byte indices all below 2^16 (with consecutive rows: gmax is below
+    // 0xffffffff) but not all staged, as in the first parts of a split
+    // layout under a fused pass, whose pairs take twice the LDS: the general
+    // gathers without the high-byte plane.
+    if constexpr (SPLIT && SWEEP && std::is_same<IdxT, Idx24>::value) {
+        if (gmax < 0x10000u) {
+            if constexpr (EARLY) descriptors();
+            load_tile<IdxT, SIGMA, EARLY, false>(a, tile, lane, c, r, flags, seg0);
+            if constexpr (!EARLY) descriptors();
+            tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN>(a, lds, ldsp, tile, lane, flags, seg0, row0,
+                                                                         segoff, open, c, r);
+            return;
+        }
+    }
     if constexpr (EARLY) descriptors();
     load_tile<IdxT, SIGMA, EARLY>(a, tile, lane, c, r, flags, seg0);
     if constexpr (!EARLY) descriptors();
