@@ -79,7 +79,6 @@ ccd::BatchLayout batch_layout(const ccdi::DevLayout& L) {
     d.tidx = T.tidx;
     d.tval = T.tval;
     d.lane_flags = T.flags;
-    d.lane_seg0 = T.seg0;
     d.tile_row0 = T.row0;
     d.tile_segoff = T.segoff;
     d.segrows = T.segrows;

@@ -56,7 +56,6 @@ void free_tiled(TiledLayout& T) {
     dfree(T.tidx);
     dfree(T.tval);
     dfree(T.flags);
-    dfree(T.seg0);
     dfree(T.row0);
     dfree(T.segoff);
     dfree(T.segrows);
@@ -412,7 +411,6 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
     const int64_t rows = split ? (int64_t)vptr.size() - 1 : L.rows;
     std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows;
     std::vector<ccd::SpanRow> span;
-    std::vector<uint16_t> seg0(ntiles * 64);
     std::vector<uint8_t> open((ntiles + 3) & ~(uint64_t)3);  // whole words: the kernels read it a word at a time
     std::vector<uint32_t> emp(rows + 1, 0u);
     for (int64_t r = 0; r < rows; ++r) {
@@ -442,13 +440,6 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
         } else {
             segoff[t] = 0xffffffffu;
         }
-        const uint32_t start0 = flags[t * 64] & 1u;
-        uint32_t cum = 0;
-        for (int l = 0; l < 64; ++l) {
-            const uint32_t f = flags[t * 64 + l];
-            seg0[t * 64 + l] = (uint16_t)(l == 0 ? 0u : cum - start0 + (f & 1u));
-            cum += (uint32_t)__builtin_popcount(f);
-        }
     }
     T.sigma = sigma;
     T.split = split;
@@ -458,7 +449,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
     T.nspan = (uint32_t)span.size();
     T.nshort = (uint32_t)std::count_if(span.begin(), span.end(), [](const ccd::SpanRow& q) { return q.t1 == q.t0 + 1; });
     int rc;
-    if ((rc = upload_vec(c, &T.flags, flags)) || (rc = upload_vec(c, &T.seg0, seg0)) ||
+    if ((rc = upload_vec(c, &T.flags, flags)) ||
         (rc = upload_vec(c, &T.row0, row0)) || (rc = upload_vec(c, &T.segoff, segoff)) ||
         (rc = upload_vec(c, &T.segrows, segrows)) || (rc = upload_vec(c, &T.open, open)) ||
         (rc = upload_vec(c, &T.spans, span)))
@@ -496,7 +487,7 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     const Launch sh = launch_shape(c, true, T.sigma, T.ib, sub, add, sweep, fused && x_is_cur);
     const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
-                     T.flags, T.seg0, T.row0, T.segoff, T.segrows, T.open, T.gmax, T.head, T.tail, L.nnz, T.ntiles,
+                     T.flags, T.row0, T.segoff, T.segrows, T.open, T.gmax, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda, T.cold_row, T.nreal,
                      sweep ? T.part : nullptr};
     {

@@ -21,7 +21,6 @@ struct TiledLayout {
     void* tidx = nullptr;
     float* tval = nullptr;
     uint32_t* flags = nullptr;
-    uint16_t* seg0 = nullptr;
     uint32_t* row0 = nullptr;
     uint32_t* segoff = nullptr;
     uint32_t* segrows = nullptr;

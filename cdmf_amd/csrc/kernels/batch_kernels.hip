@@ -96,7 +96,6 @@ __device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const Pas
     a.ya = v.ya;
     a.xp = L.pair;
     a.lane_flags = L.lane_flags;
-    a.lane_seg0 = L.lane_seg0;
     a.tile_row0 = L.tile_row0;
     a.tile_segoff = L.tile_segoff;
     a.segrows = L.segrows;

@@ -258,7 +258,6 @@ hipError_t launch_copy(const float4* src, float4* dst, uint64_t count, hipStream
 // (i/4)*256 + 4*l + i%4) so a 16-byte load of every lane reads 1 KiB of
 // consecutive values.
 //   lane_flags  bit i: element i of the lane is the first element of a row
-//   lane_seg0   segment (row of the tile, counted from 0) of the lane's element 0
 //   tile_row0   row of the tile's first element
 //   tile_segoff 0xffffffff when the tile's rows are consecutive, else offset
 //               into segrows (row of every segment of the tile)
@@ -282,7 +281,6 @@ struct TiledArgs {
     const void* ya;
     const void* xp;
     const uint32_t* lane_flags;
-    const uint16_t* lane_seg0;
     const uint32_t* tile_row0;
     const uint32_t* tile_segoff;
     const uint32_t* segrows;
@@ -393,7 +391,6 @@ struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
     const void* tidx;
     float* tval;
     const uint32_t* lane_flags;
-    const uint16_t* lane_seg0;
     const uint32_t* tile_row0;
     const uint32_t* tile_segoff;
     const uint32_t* segrows;

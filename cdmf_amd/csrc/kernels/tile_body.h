@@ -88,6 +88,22 @@ struct HiPlane<Idx24, SIGMA> {
     __device__ __forceinline__ uint32_t at(int i) const { return ((w[i >> 2] >> (8 * (i & 3))) & 0xffu) << 16; }
 };
 
+// Segment (row of the tile, counted from 0) of a lane's element 0, from the
+// lanes' row-start flags: the row starts in earlier lanes (a wave prefix sum
+// of popcounts), less one when the tile's first element starts a row, plus
+// one when the lane's element 0 starts a row; lane 0 is segment 0.
+__device__ __forceinline__ uint32_t lane_seg0(uint32_t flags, int lane) {
+    const uint32_t pc = (uint32_t)__popc(flags);
+    uint32_t incl = pc;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const uint32_t start0 = (uint32_t)__shfl(flags, 0) & 1u;
+    return lane == 0 ? 0u : incl - pc - start0 + (flags & 1u);
+}
+
 // The lane's streams of one tile, unpacked: SIGMA indices and values, its
 // row-start flags and first segment.  Indices are loaded first: the gathers
 // wait for them, not for the values.  HI: the high bytes of the 3-byte
@@ -102,9 +118,8 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
     const IE* __restrict__ tidx = static_cast<const IE*>(a.tidx) + (size_t)tile * TS;
     const float* __restrict__ tval = a.tval + (size_t)tile * TS;
     flags = a.lane_flags[(size_t)tile * WAVE + lane];
-    seg0 = a.lane_seg0[(size_t)tile * WAVE + lane];
-    // ORDERED: keep these two loads ahead of the streams (the row lookups
-    // wait for them, and loads complete in issue order)
+    // ORDERED: keep this load ahead of the streams (the row lookups wait for
+    // it, and loads complete in issue order)
     if constexpr (ORDERED) __builtin_amdgcn_sched_barrier(0);
     vu4 ci[SIGMA / G];
 #pragma unroll
@@ -130,6 +145,7 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
             else c[4 * q + j] = wd[j];
         }
     }
+    seg0 = lane_seg0(flags, lane);
 }
 
 // One tile after its streams are loaded: the lane's per-row vectors and
