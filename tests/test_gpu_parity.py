@@ -569,6 +569,48 @@ def test_fp64_sums_against_fp64_oracle(oracle_lib, variant):
     _compare(*got, W64, H64, rm64, r64, k64, "fp64-sums v%d" % variant)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "f64"])
+def test_split_rows_precisions(oracle_lib, prec):
+    """The split tile layout in the other precisions, against the oracle's
+    twin of that precision, beside the plain tile layout.  BF16 storage:
+    the LDS prefix holds twice the entries (81,408 > the split at 40,704), so
+    second-part tiles mix LDS and global gathers; the bar is test_bf16_storage's
+    (RMSE 1e-3, W/H 1e-2 Frobenius).  FP64 sums: (g, h) partials are doubles;
+    W, H and the residual to the FP64 oracle at the full-size bar
+    (_close_bulk), RMSE within 1e-4."""
+    from cdmf_amd import data as D
+    solver = _solver()
+    ds = D.gen_powerlaw(m=90000, n=2000, nnz_train=300000, nnz_test=15000, max_row_degree=2000, rating_kind=1,
+                        seed=23)
+    k, lam, outer, inner = 4, 0.05, 2, 3
+    lam32 = float(np.float32(lam))
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    code = solver.PREC_BF16 if prec == "bf16" else solver.PREC_FP64_SUMS
+    Wr, Hr, rmr, resr, _ = oracle_lib.ccd(ds, W0, H0, k, lam32, outer, inner, dtype=prec)
+    for split in (1, 0):
+        _say("case split_rows=%d precision %s: m=%d nnz=%d" % (split, prec, ds.m, ds.nnz))
+        with solver.Solver(0) as s:
+            s.set_plan(split_rows=split)
+            s.upload(ds)
+            s.set_factors(W0, H0)
+            res = s.run(k, lam, outer, inner, variant=3, precision=code)
+            W, H = s.factors()
+            rcsr, _ = s.residual()
+        if prec == "bf16":
+            for X in (W, H):
+                assert np.all((X.view(np.uint32) & 0xFFFF) == 0)
+            np.testing.assert_allclose(res.rmse, rmr, rtol=0, atol=1e-3)
+            for X, R, what in ((W, Wr, "W"), (H, Hr, "H")):
+                f = np.linalg.norm(X - R) / np.linalg.norm(R)
+                _say("split_rows=%d bf16 %s Frobenius vs bf16 oracle %.3g" % (split, what, f))
+                assert f <= 1e-2, what
+        else:
+            np.testing.assert_allclose(res.rmse, rmr, rtol=0, atol=RMSE_ATOL)
+            _close_bulk(W, Wr, "split_rows=%d f64 W" % split)
+            _close_bulk(H, Hr, "split_rows=%d f64 H" % split)
+            _close_bulk(rcsr, resr, "split_rows=%d f64 residual" % split)
+
+
 @pytest.mark.skipif(os.environ.get("CDMF_SKIP_LARGE") == "1", reason="large case disabled by CDMF_SKIP_LARGE=1")
 def test_netflix_shape_fp64_sums(oracle_lib):
     """The full-size case in the FP64-sums check mode: W, H and residual to
