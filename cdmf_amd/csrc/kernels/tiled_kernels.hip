@@ -24,8 +24,12 @@
 // fixed order); rows crossing tile boundaries leave per-tile partials
 // (head/tail) that tiled_span_kernel adds up in tile order.  A row's length
 // comes from the row-start flags (its start and end positions in the tile).
-// No atomics: results are run-to-run deterministic.  Built with
-// -ffp-contract=off like the binned kernels.
+// Tiles whose gathers all fall in the staged prefix (per-tile largest index)
+// take a copy of the body with plain LDS reads.  A layout whose gathered
+// vector exceeds the prefix can hold its rows split in two virtual rows
+// (first the parts gathered from the prefix, then the rest), finished by
+// split_finish_kernel.  No atomics: results are run-to-run deterministic.
+// Built with -ffp-contract=off like the binned kernels.
 #include "tile_body.h"
 
 namespace ccd {

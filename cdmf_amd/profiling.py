@@ -58,7 +58,7 @@ def kernel_class(name: str):
             return "copy_probe"
         if "commit_kernel" in name:
             return "copy"
-        if "span_kernel" in name or "finalize_kernel" in name or "combine_kernel" in name:
+        if "span_kernel" in name or "finalize_kernel" in name or "split_finish_kernel" in name:
             return "finalize"
         if "rmse" in name:
             return "rmse"

@@ -194,7 +194,7 @@ typedef struct ccd_plan_params {
                                  1 or 2 = at most that many; the staged LDS prefix is sized to match */
     uint32_t split_rows;      /* tiled sweeps, layouts whose gathered vector exceeds the LDS prefix: 1 = keep
                                  each row's elements gathered from beyond the prefix in tiles of their own,
-                                 finished by a combine pass (default); 0 = one tile range in row order */
+                                 finished by a per-row finish pass (default); 0 = one tile range in row order */
 } ccd_plan_params;
 
 int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);

@@ -11,6 +11,6 @@ def test_kernel_classes():
     assert kernel_class(t % ("unsigned short", "true", "false", "true", "false")) == "fused_s"
     assert kernel_class(t % ("ccd::Idx24", "true", "false", "false", "false")) == "residual"
     assert kernel_class("void ccd::(anonymous namespace)::tiled_span_kernel<float, float>(ccd::SpanArgs)") == "finalize"
-    assert kernel_class("void ccd::(anonymous namespace)::combine_kernel<float, float>(ccd::CombineArgs)") == "finalize"
+    assert kernel_class("void ccd::(anonymous namespace)::split_finish_kernel<float, float>(ccd::SplitArgs)") == "finalize"
     assert kernel_class("void ccd::(anonymous namespace)::commit_kernel<float>(ccd::CommitArgs)") == "copy"
     assert kernel_class("__amd_rocclr_copyBuffer") is None
