@@ -181,10 +181,10 @@ __device__ __forceinline__ Staged stage_pass(const BatchLayout& L, const PassVec
         st.hot_p = min(L.gdim, cap / 2);
         const FPair<float>* src = static_cast<const FPair<float>*>(L.pair);
         FPair<float>* dst = reinterpret_cast<FPair<float>*>(lds);
-        for (uint32_t i = threadIdx.x; i < st.hot_p; i += BATCH_BLOCK) dst[i] = src[i];
+        stage_lds<BATCH_BLOCK>(dst, src, st.hot_p);
     } else {
         st.hot_x = min(L.gdim, cap);
-        for (uint32_t i = threadIdx.x; i < st.hot_x; i += BATCH_BLOCK) lds[i] = v.x[i];
+        stage_lds<BATCH_BLOCK>(lds, v.x, st.hot_x);
     }
     __syncthreads();
     return st;

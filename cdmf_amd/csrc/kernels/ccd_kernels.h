@@ -57,6 +57,24 @@ __device__ __forceinline__ T ld_const(const T* p, size_t i) {
     return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
+// Copy of n elements into LDS by a block of BLOCK threads.  Each thread
+// issues a round of up to U loads before its first LDS write, so a round
+// costs one memory round trip rather than one per element (the plain loop
+// waits for every load before its write).
+template <int BLOCK, class T>
+__device__ __forceinline__ void stage_lds(T* __restrict__ dst, const T* __restrict__ src, uint32_t n) {
+    constexpr int U = sizeof(T) >= 16 ? 4 : sizeof(T) >= 8 ? 8 : 16;
+    for (uint32_t base = threadIdx.x; base < n; base += U * BLOCK) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u * BLOCK < n) v[u] = src[base + u * BLOCK];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u * BLOCK < n) dst[base + u * BLOCK] = v[u];
+    }
+}
+
 // A pair is read as one word (8 bytes for FP32, 4 for BF16) and split.
 template <class TF>
 using PairWord = typename std::conditional<sizeof(TF) == 4, uint64_t, uint32_t>::type;

@@ -45,8 +45,8 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TF* lds = reinterpret_cast<TF*>(lds_raw);
     FPair<TF>* ldsp = reinterpret_cast<FPair<TF>*>(lds_raw + lds_pair_offset<TF>(a.hot_x));
-    for (uint32_t i = threadIdx.x; i < a.hot_x; i += SWEEP_BLOCK) lds[i] = static_cast<const TF*>(a.x)[i];
-    for (uint32_t i = threadIdx.x; i < a.hot_p; i += SWEEP_BLOCK) ldsp[i] = static_cast<const FPair<TF>*>(a.xp)[i];
+    stage_lds<SWEEP_BLOCK>(lds, static_cast<const TF*>(a.x), a.hot_x);
+    stage_lds<SWEEP_BLOCK>(ldsp, static_cast<const FPair<TF>*>(a.xp), a.hot_p);
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
