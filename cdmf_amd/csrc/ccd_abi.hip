@@ -517,7 +517,9 @@ std::vector<uint32_t> degree_order(const uint32_t* ptr, int64_t rows) {
 
 // Element order inside each row of a renumbered layout: the caller's, sorted
 // by internal index, or split into the elements whose internal index is
-// below a bound followed by the others (each part in the caller's order).
+// below a bound, in the caller's order, followed by the others sorted by
+// internal index (their gathers miss the LDS prefix; sorted, a lane's
+// consecutive elements share cache lines).
 enum class RowOrder { keep, sort, split };
 
 // One compressed layout in internal numbering: rows in `perm` order, indices
@@ -548,7 +550,7 @@ void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const floa
                 for (uint32_t q = 0; q < len; ++q) {
                     const uint32_t k = ipos[idx[e0 + q]];
                     nb += k < at ? 1u : 0u;
-                    key[q] = (uint64_t)(order == RowOrder::sort ? k : (k >= at ? 1u : 0u)) << 32 | q;
+                    key[q] = (uint64_t)(order == RowOrder::sort || k >= at ? k : 0u) << 32 | q;
                 }
                 if (order != RowOrder::keep) std::sort(key.begin(), key.end());
                 if (order == RowOrder::split) below[i] = nb;
