@@ -231,8 +231,8 @@ int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool tri
         BATCH_TRY(b, ccdi::dalloc(&b->d_block_problem, blocks));
         b->cap_blocks = blocks;
     }
-    BATCH_TRY(b, hipMemcpy(b->d_probs, probs.data(), sizeof(ccd::BatchProblem) * np, hipMemcpyHostToDevice));
-    BATCH_TRY(b, hipMemcpy(b->d_block_problem, owner.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice));
+    BATCH_TRY(b, ccd::copy_sync(b->stream, b->d_probs, probs.data(), sizeof(ccd::BatchProblem) * np, hipMemcpyHostToDevice));
+    BATCH_TRY(b, ccd::copy_sync(b->stream, b->d_block_problem, owner.data(), sizeof(uint32_t) * blocks, hipMemcpyHostToDevice));
     args->probs = b->d_probs;
     args->block_problem = b->d_block_problem;
     args->nprob = (uint32_t)np;

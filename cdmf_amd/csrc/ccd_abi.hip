@@ -312,24 +312,24 @@ int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t
     L.hptr.assign(ptr, ptr + rows + 1);
     L.split_cap = 0;
     HIP_TRY(c, dalloc(&L.ptr, rows + 1));
-    HIP_TRY(c, hipMemcpy(L.ptr, ptr, sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, ccd::copy_sync(c->stream, L.ptr, ptr, sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice));
     if (L.idx16) {
         std::vector<uint16_t> tmp(nnz);
         for (uint64_t e = 0; e < nnz; ++e) tmp[e] = (uint16_t)idx[e];
         uint16_t* d = nullptr;
         HIP_TRY(c, dalloc(&d, nnz));
         L.idx = d;
-        HIP_TRY(c, hipMemcpy(d, tmp.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, d, tmp.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice));
     } else {
         uint32_t* d = nullptr;
         HIP_TRY(c, dalloc(&d, nnz));
         L.idx = d;
-        HIP_TRY(c, hipMemcpy(d, idx, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, d, idx, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
     }
     HIP_TRY(c, dalloc(&L.val, nnz));
     HIP_TRY(c, dalloc(&L.val0, nnz));
-    HIP_TRY(c, hipMemcpy(L.val0, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(L.val, L.val0, sizeof(float) * nnz, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, ccd::copy_sync(c->stream, L.val0, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
+    HIP_TRY(c, ccd::copy_sync(c->stream, L.val, L.val0, sizeof(float) * nnz, hipMemcpyDeviceToDevice));
 
     HostPlan P = make_plan(ptr, rows, c->plan);
     if (P.split_beg.back() >= (1u << 24) || P.units.size() >= 0xffffffffull)
@@ -339,13 +339,13 @@ int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t
     L.npieces = P.split_beg.back();
     HIP_TRY(c, dalloc(&L.units, P.units.size()));
     if (!P.units.empty())
-        HIP_TRY(c, hipMemcpy(L.units, P.units.data(), sizeof(uint4) * P.units.size(), hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, L.units, P.units.data(), sizeof(uint4) * P.units.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, dalloc(&L.split_rows, P.split_rows.size()));
     if (!P.split_rows.empty())
-        HIP_TRY(c, hipMemcpy(L.split_rows, P.split_rows.data(), sizeof(uint32_t) * P.split_rows.size(),
-                             hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, L.split_rows, P.split_rows.data(), sizeof(uint32_t) * P.split_rows.size(),
+                                  hipMemcpyHostToDevice));
     HIP_TRY(c, dalloc(&L.split_beg, P.split_beg.size()));
-    HIP_TRY(c, hipMemcpy(L.split_beg, P.split_beg.data(), sizeof(uint32_t) * P.split_beg.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, ccd::copy_sync(c->stream, L.split_beg, P.split_beg.data(), sizeof(uint32_t) * P.split_beg.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, dalloc(&L.partials, L.npieces));
     return CCD_OK;
 }
@@ -353,7 +353,7 @@ int upload_layout(ccd_ctx* c, DevLayout& L, int64_t rows, int64_t gdim, uint64_t
 template <class T>
 int upload_vec(ccd_ctx* c, T** dst, const std::vector<T>& v) {
     HIP_TRY(c, dalloc(dst, v.size()));
-    if (!v.empty()) HIP_TRY(c, hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    if (!v.empty()) HIP_TRY(c, ccd::copy_sync(c->stream, *dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return CCD_OK;
 }
 
@@ -774,9 +774,9 @@ int32_t ccd_upload_test(ccd_ctx* c, uint64_t nnz, const uint32_t* row, const uin
             r2[e] = rpos[row[e]];
             c2[e] = cpos[col[e]];
         }
-        HIP_TRY(c, hipMemcpy(c->trow, r2.data(), 4 * nnz, hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(c->tcol, c2.data(), 4 * nnz, hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(c->tval, val, 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, c->trow, r2.data(), 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, c->tcol, c2.data(), 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, c->tval, val, 4 * nnz, hipMemcpyHostToDevice));
     }
     return CCD_OK;
 }
@@ -800,13 +800,13 @@ int32_t ccd_set_factors(ccd_ctx* c, uint32_t k, const float* W, const float* H) 
         std::vector<float> t2((size_t)k * std::max(c->m, c->n));
         for (uint32_t t = 0; t < k; ++t)
             for (int64_t i = 0; i < c->m; ++i) t2[(size_t)t * c->m + i] = W[(size_t)t * c->m + c->csr.perm[i]];
-        HIP_TRY(c, hipMemcpy(c->W, t2.data(), sizeof(float) * (size_t)k * c->m, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, c->W, t2.data(), sizeof(float) * (size_t)k * c->m, hipMemcpyHostToDevice));
         for (uint32_t t = 0; t < k; ++t)
             for (int64_t j = 0; j < c->n; ++j) t2[(size_t)t * c->n + j] = H[(size_t)t * c->n + c->csc.perm[j]];
-        HIP_TRY(c, hipMemcpy(c->H, t2.data(), sizeof(float) * (size_t)k * c->n, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, c->H, t2.data(), sizeof(float) * (size_t)k * c->n, hipMemcpyHostToDevice));
     }
-    HIP_TRY(c, hipMemcpy(c->csr.val, c->csr.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
-    HIP_TRY(c, hipMemcpy(c->csc.val, c->csc.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, ccd::copy_sync(c->stream, c->csr.val, c->csr.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, ccd::copy_sync(c->stream, c->csc.val, c->csc.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
     c->factors = true;
     c->continued = false;
     return CCD_OK;

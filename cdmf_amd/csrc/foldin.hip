@@ -239,7 +239,7 @@ int32_t ccd_mlp_create(int32_t device, uint32_t in, uint32_t hidden, uint32_t ou
               dalloc(&m->out_mul, out) == hipSuccess && dalloc(&m->tgt_sub, out) == hipSuccess &&
               dalloc(&m->tgt_mul, out) == hipSuccess && dalloc(&m->part, ccd::MSE_PARTS) == hipSuccess &&
               dalloc(&m->gpart, m->gpart_cap) == hipSuccess &&
-              hipMemcpy(m->P, P.data(), sizeof(float) * m->np, hipMemcpyHostToDevice) == hipSuccess &&
+              ccd::copy_sync(m->stream, m->P, P.data(), sizeof(float) * m->np, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemset(m->mom, 0, sizeof(float) * m->np) == hipSuccess &&
               hipMemset(m->vel, 0, sizeof(float) * m->np) == hipSuccess;
     if (!ok) {
@@ -279,7 +279,7 @@ int32_t ccd_mlp_set_params(ccd_mlp* m, const float* P) {
     if (!m || !P) return CCD_ERR_INVALID;
     OBJ_TRY(m, hipSetDevice(m->device));
     OBJ_TRY(m, hipStreamSynchronize(m->stream));
-    OBJ_TRY(m, hipMemcpy(m->P, P, sizeof(float) * m->np, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->P, P, sizeof(float) * m->np, hipMemcpyHostToDevice));
     m->p16_valid = false;
     return CCD_OK;
 }
@@ -305,12 +305,12 @@ int32_t ccd_mlp_set_norm(ccd_mlp* m, const float* in_mean, const float* in_std, 
     }
     OBJ_TRY(m, hipSetDevice(m->device));
     OBJ_TRY(m, hipStreamSynchronize(m->stream));
-    OBJ_TRY(m, hipMemcpy(m->in_sub, a.data(), 4 * m->in, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->in_mul, b.data(), 4 * m->in, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->out_sub, c.data(), 4 * m->out, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->out_mul, d.data(), 4 * m->out, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->tgt_sub, e.data(), 4 * m->out, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->tgt_mul, g.data(), 4 * m->out, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->in_sub, a.data(), 4 * m->in, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->in_mul, b.data(), 4 * m->in, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->out_sub, c.data(), 4 * m->out, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->out_mul, d.data(), 4 * m->out, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->tgt_sub, e.data(), 4 * m->out, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->tgt_mul, g.data(), 4 * m->out, hipMemcpyHostToDevice));
     m->has_norm = true;
     return CCD_OK;
 }
@@ -326,7 +326,7 @@ int32_t ccd_mlp_forward(ccd_mlp* m, uint64_t rows, const float* X, float* Y, int
         return obj_err(m->err, CCD_ERR_ALLOC, "output buffer");
     }
     int rc = CCD_OK;
-    if (hipMemcpy(dX, X, 4 * rows * m->in, hipMemcpyHostToDevice) != hipSuccess) rc = CCD_ERR_HIP;
+    if (ccd::copy_sync(m->stream, dX, X, 4 * rows * m->in, hipMemcpyHostToDevice) != hipSuccess) rc = CCD_ERR_HIP;
     if (!rc) rc = mlp_forward_dev(m, rows, dX, dYo, bf16_weights != 0, m->stream);
     if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = CCD_ERR_HIP;
     if (!rc && hipMemcpy(Y, dYo, 4 * rows * m->out, hipMemcpyDeviceToHost) != hipSuccess) rc = CCD_ERR_HIP;
@@ -341,8 +341,8 @@ int32_t ccd_mlp_loss_grad(ccd_mlp* m, uint64_t rows, const float* Xn, const floa
     OBJ_TRY(m, hipSetDevice(m->device));
     if (int rc = mlp_workspace(m, rows)) return rc;
     if (!m->loss) OBJ_TRY(m, dalloc(&m->loss, 1));
-    OBJ_TRY(m, hipMemcpy(m->Xn, Xn, 4 * rows * m->in, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->Tn, Tn, 4 * rows * m->out, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->Xn, Xn, 4 * rows * m->in, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->Tn, Tn, 4 * rows * m->out, hipMemcpyHostToDevice));
     if (int rc = mlp_forward_ws(m, (uint32_t)rows, false, m->stream)) return rc;
     if (int rc = mlp_backward_ws(m, (uint32_t)rows, m->loss, m->stream)) return rc;
     OBJ_TRY(m, hipStreamSynchronize(m->stream));
@@ -363,8 +363,8 @@ int32_t ccd_mlp_train(ccd_mlp* m, uint64_t rows, const float* X, const float* T,
     OBJ_TRY(m, ensure(&m->perm, &m->cap_perm, rows));
     const uint64_t nb = (rows + batch - 1) / batch;
     OBJ_TRY(m, ensure(&m->loss, &m->cap_loss, nb));
-    OBJ_TRY(m, hipMemcpy(m->X, X, 4 * rows * m->in, hipMemcpyHostToDevice));
-    OBJ_TRY(m, hipMemcpy(m->T, T, 4 * rows * m->out, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->X, X, 4 * rows * m->in, hipMemcpyHostToDevice));
+    OBJ_TRY(m, ccd::copy_sync(m->stream, m->T, T, 4 * rows * m->out, hipMemcpyHostToDevice));
     std::vector<uint32_t> perm(rows);
     std::vector<double> losses(nb);
     uint64_t st = seed ^ 0x747261696eull;
@@ -374,7 +374,7 @@ int32_t ccd_mlp_train(ccd_mlp* m, uint64_t rows, const float* X, const float* T,
         std::iota(perm.begin(), perm.end(), 0u);
         for (uint64_t i = rows - 1; i > 0; --i) std::swap(perm[i], perm[splitmix(st) % (i + 1)]);
         OBJ_TRY(m, hipStreamSynchronize(s));
-        OBJ_TRY(m, hipMemcpy(m->perm, perm.data(), 4 * rows, hipMemcpyHostToDevice));
+        OBJ_TRY(m, ccd::copy_sync(m->stream, m->perm, perm.data(), 4 * rows, hipMemcpyHostToDevice));
         for (uint64_t q = 0; q < nb; ++q) {
             const uint32_t n = (uint32_t)std::min<uint64_t>(batch, rows - q * batch);
             const uint32_t* pr = m->perm + q * batch;
@@ -438,12 +438,12 @@ int fold_upload(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* 
     OBJ_TRY(f, ensure(&f->val, &f->cap_val, nnz));
     OBJ_TRY(f, ensure(&f->rbuf, &f->cap_rbuf, nnz));
     OBJ_TRY(f, ensure(&f->long_rows, &f->cap_long, lr.size()));
-    OBJ_TRY(f, hipMemcpy(f->ptr, ptr, 4 * ((size_t)rows + 1), hipMemcpyHostToDevice));
-    if (!lr.empty()) OBJ_TRY(f, hipMemcpy(f->long_rows, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice));
+    OBJ_TRY(f, ccd::copy_sync(f->stream, f->ptr, ptr, 4 * ((size_t)rows + 1), hipMemcpyHostToDevice));
+    if (!lr.empty()) OBJ_TRY(f, ccd::copy_sync(f->stream, f->long_rows, lr.data(), 4 * lr.size(), hipMemcpyHostToDevice));
     f->nlong = (uint32_t)lr.size();
     if (nnz) {
-        OBJ_TRY(f, hipMemcpy(f->idx, idx, 4 * nnz, hipMemcpyHostToDevice));
-        OBJ_TRY(f, hipMemcpy(f->val, val, 4 * nnz, hipMemcpyHostToDevice));
+        OBJ_TRY(f, ccd::copy_sync(f->stream, f->idx, idx, 4 * nnz, hipMemcpyHostToDevice));
+        OBJ_TRY(f, ccd::copy_sync(f->stream, f->val, val, 4 * nnz, hipMemcpyHostToDevice));
     }
     *a = ccd::FoldArgs{f->ptr, f->idx, f->val, f->H, f->HT, (uint32_t)rows, f->k, (uint32_t)f->n, 0, f->lambda};
     return CCD_OK;
@@ -469,7 +469,7 @@ int32_t ccd_fold_create(int32_t device, uint32_t k, int64_t n, const float* H, f
     bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&f->ea) == hipSuccess && hipEventCreate(&f->eb) == hipSuccess &&
               dalloc(&f->H, kn) == hipSuccess && dalloc(&f->HT, kn) == hipSuccess &&
-              hipMemcpy(f->H, H, 4 * kn, hipMemcpyHostToDevice) == hipSuccess &&
+              ccd::copy_sync(f->stream, f->H, H, 4 * kn, hipMemcpyHostToDevice) == hipSuccess &&
               ccd::launch_transpose(f->H, f->HT, k, (uint32_t)n, f->stream) == hipSuccess &&
               hipStreamSynchronize(f->stream) == hipSuccess;
     if (!ok) {
@@ -518,7 +518,7 @@ int32_t ccd_fold_rows(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint
     OBJ_TRY(f, ensure(&f->W, &f->cap_w, nw));
     if (W0) {
         OBJ_TRY(f, ensure(&f->W0, &f->cap_w0, nw));
-        if (nw) OBJ_TRY(f, hipMemcpy(f->W0, W0, 4 * nw, hipMemcpyHostToDevice));
+        if (nw) OBJ_TRY(f, ccd::copy_sync(f->stream, f->W0, W0, 4 * nw, hipMemcpyHostToDevice));
     }
     OBJ_TRY(f, hipEventRecord(f->ea, f->stream));
     OBJ_TRY(f, ccd::launch_fold_sweeps(a, f->long_rows, f->nlong, W0 ? f->W0 : nullptr, f->W, f->rbuf, f->stream));

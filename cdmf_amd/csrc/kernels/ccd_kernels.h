@@ -75,6 +75,17 @@ __device__ __forceinline__ void stage_lds(T* __restrict__ dst, const T* __restri
     }
 }
 
+// Copy made by a set-up call (host to device, device to device): ordered on
+// the object's stream s and complete when it returns.  The library's streams
+// are non-blocking, so their kernels do not wait for copies on the null
+// stream, and a device-to-device hipMemcpy may return before its copy ends.
+inline hipError_t copy_sync(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (bytes == 0) return hipSuccess;
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    const hipError_t w = hipStreamSynchronize(s);
+    return e != hipSuccess ? e : w;
+}
+
 // A pair is read as one word (8 bytes for FP32, 4 for BF16) and split.
 template <class TF>
 using PairWord = typename std::conditional<sizeof(TF) == 4, uint64_t, uint32_t>::type;

@@ -152,6 +152,33 @@ def test_split_rows_and_groups(oracle_lib):
         _compare(*got, *ref, "plan %s" % plan)
 
 
+def test_context_reuse_back_to_back():
+    """One context, several solves from the same start with nothing between
+    ccd_set_factors and ccd_run: set-up copies (the residual reset is a
+    device-to-device copy of every rating) must be complete before the run's
+    kernels read them, whatever the launch mode.  Every solve is bitwise the
+    first one."""
+    from cdmf_amd import data as D
+    from cdmf_amd import solver
+    ds = D.gen_powerlaw(m=120000, n=9000, nnz_train=12000000, nnz_test=200000, max_row_degree=4000,
+                        col_exponent=1.0, rating_kind=1, seed=5)
+    k, lam = 8, 0.05
+    W0 = D.initial_col(k, ds.m)
+    H0 = np.zeros((k, ds.n), np.float32)
+    _say("case context reuse: m=%d n=%d nnz=%d" % (ds.m, ds.n, ds.nnz))
+    with solver.Solver(0) as s:
+        s.upload(ds)
+        out = []
+        for mode in ("eager", "graph", "profile", "eager", "graph", "eager"):
+            s.set_factors(W0, H0)
+            r = s.run(k, lam, 3, 3, variant=3, use_graph=mode == "graph", profile=mode == "profile",
+                      profile_kinds=("sweep_f",) if mode == "profile" else ())
+            out.append((mode, r.rmse, *s.factors()))
+    for mode, rmse, W, H in out[1:]:
+        assert rmse == out[0][1], "%s: test RMSE %s differs from the first solve's %s" % (mode, rmse, out[0][1])
+        assert np.array_equal(W, out[0][2]) and np.array_equal(H, out[0][3]), mode
+
+
 def test_deterministic(oracle_lib):
     ds = _small()
     W0, H0 = oracle_lib.initial_col(8, ds.m), oracle_lib.initial_col(8, ds.n)
