@@ -147,7 +147,9 @@ def main():
     # The warmup times every kernel class (event records between kernels add
     # a few microseconds each); the timed run times only the dominant sweep
     # kernel, with events bound to its own dispatches, which add nothing
-    # between kernels.
+    # between kernels.  A bound pair still costs about 5 us of device time,
+    # so the library times one launch in 5 of that kernel (the launches are
+    # identical passes over the same layout).
     sweep_kinds = ("sweep_f", "sweep_s", "fused_f", "fused_s")
     all_stats = None
     timed_kinds = sweep_kinds

@@ -153,7 +153,10 @@ hipEvent_t take_event(ccd_ctx* c) {
 // Times the launches made in its scope as one record of `kind` when that
 // kind is profiled.  Default: event records before and after.  bound: the
 // events are handed to a single launch (ev()) that binds them to its own
-// dispatch, so nothing is added between kernels.
+// dispatch, so nothing is added between kernels; a bound pair still costs
+// about 5 us of device time, so when the caller selects kinds
+// (profile_kinds) one bound launch in PROF_SAMPLE of each is timed.
+constexpr uint64_t PROF_SAMPLE = 5;
 struct Timed {
     ccd_ctx* c;
     int kind;
@@ -161,6 +164,7 @@ struct Timed {
     hipEvent_t a = nullptr, b = nullptr;
     Timed(ccd_ctx* ctx, int k, bool bind = false) : c(ctx), kind(k), bound(bind) {
         if (!c->prof || (k >= 0 && c->prof_kinds && !((c->prof_kinds >> k) & 1u))) return;
+        if (bound && k >= 0 && c->prof_kinds && c->kind_seen[k]++ % PROF_SAMPLE != 0) return;
         a = take_event(c);
         b = take_event(c);
         if (!a || !b) {
@@ -881,6 +885,7 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     for (int q = 0; q < CCD_K_COUNT; ++q) {
         c->kind_ms[q] = 0.0;
         c->kind_n[q] = 0;
+        c->kind_seen[q] = 0;
     }
     const bool tl = p->variant == CCD_VARIANT_TILED, sp = c->plan.split_rows != 0;
     c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, tl, sp, false, 1, 0, true);

@@ -133,6 +133,7 @@ struct ccd_ctx {
     size_t pool_used = 0;
     double kind_ms[CCD_K_COUNT] = {};
     uint64_t kind_n[CCD_K_COUNT] = {};
+    uint64_t kind_seen[CCD_K_COUNT] = {};  // bound launches of each kind this run (timed or not)
     double kind_bytes[CCD_K_COUNT] = {};
     double loop_s = 0.0;
     // outer iteration captured for replay (use_graph), valid for graph_key

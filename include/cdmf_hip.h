@@ -82,7 +82,9 @@ typedef struct ccd_params {
     int32_t use_graph;    /* 1: replay each outer iteration from a captured HIP graph */
     int32_t profile_kinds; /* with profile = 1: bitmask of (1 << ccd_kernel_kind) to time, 0 = every kind.
                               Sweep kernels are timed by events bound to their own dispatch, which adds
-                              nothing between kernels; the other kinds by event records around the launch,
+                              nothing between kernels but about 5 us of device time per timed launch, so
+                              with profile_kinds != 0 one launch in 5 of each selected sweep kind is timed
+                              (the 1st, 6th, ...); the other kinds by event records around the launch,
                               which add a few microseconds between kernels each. */
     int32_t reserved[4];  /* zero */
 } ccd_params;

@@ -222,7 +222,8 @@ def test_graph_replay_bitwise(oracle_lib, variant):
 def test_profile_kinds():
     """profile_kinds times only the chosen kernel classes; the counts follow
     the loop: the very first sweep of a run is plain, every other rank's
-    first sweep is fused."""
+    first sweep is fused.  With kinds selected, one launch in 5 of a sweep
+    kind is timed."""
     solver = _solver()
     ds = _small()
     k, outer, inner = 6, 2, 3
@@ -235,7 +236,8 @@ def test_profile_kinds():
             s.run(k, 0.05, outer, inner, variant=3, profile=True, profile_kinds=kinds)
             st = s.kernel_stats()
             _say("profile_kinds %s: %s" % (kinds, {kk: v["launches"] for kk, v in st.items()}))
-            assert st["sweep_f"]["launches"] == outer * k * (inner - 1) + 1
+            nf = outer * k * (inner - 1) + 1
+            assert st["sweep_f"]["launches"] == ((nf + 4) // 5 if kinds else nf)
             assert st["sweep_f"]["total_ms"] > 0
             if kinds:
                 assert all(v["launches"] == 0 for kk, v in st.items() if kk != "sweep_f")
