@@ -240,8 +240,9 @@ int32_t ccd_mlp_create(int32_t device, uint32_t in, uint32_t hidden, uint32_t ou
               dalloc(&m->tgt_mul, out) == hipSuccess && dalloc(&m->part, ccd::MSE_PARTS) == hipSuccess &&
               dalloc(&m->gpart, m->gpart_cap) == hipSuccess &&
               ccd::copy_sync(m->stream, m->P, P.data(), sizeof(float) * m->np, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemset(m->mom, 0, sizeof(float) * m->np) == hipSuccess &&
-              hipMemset(m->vel, 0, sizeof(float) * m->np) == hipSuccess;
+              ccd::launch_fill(m->mom, 0.0f, m->np, m->stream) == hipSuccess &&
+              ccd::launch_fill(m->vel, 0.0f, m->np, m->stream) == hipSuccess &&
+              hipStreamSynchronize(m->stream) == hipSuccess;
     if (!ok) {
         ccd_mlp_destroy(m);
         return CCD_ERR_HIP;
