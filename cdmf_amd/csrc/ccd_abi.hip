@@ -458,8 +458,10 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
         (rc = upload_vec(c, &T.segrows, segrows)) || (rc = upload_vec(c, &T.open, open)) ||
         (rc = upload_vec(c, &T.spans, span)))
         return rc;
+    T.cold_tile0 = 0xffffffffu;
     if (split) {
         T.nreal = (uint32_t)L.rows;
+        T.cold_tile0 = (uint32_t)((vptr[L.rows] + TS - 1) / TS);
         if ((rc = upload_vec(c, &T.vpos, vpos)) || (rc = upload_vec(c, &T.cold_row, cold_row)) ||
             (rc = upload_vec(c, &T.srows, srows)))
             return rc;
@@ -493,7 +495,7 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      T.flags, T.row0, T.segoff, T.segrows, T.open, T.gmax, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda, T.cold_row, T.nreal,
-                     sweep ? T.part : nullptr};
+                     sweep ? T.part : nullptr, T.cold_tile0};
     {
         Timed tm(c, kind, true);
         HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, T.ib, sub, add, sweep, sh.blocks, c->stream, tm.ev()));

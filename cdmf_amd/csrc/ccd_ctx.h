@@ -39,6 +39,7 @@ struct TiledLayout {
     // finishes it.
     bool split = false;
     uint32_t nreal = 0;
+    uint32_t cold_tile0 = 0xffffffffu;  // first tile holding only second parts (split layouts)
     uint32_t* vpos = nullptr;      // position in the layout of each element, in tiled order
     uint32_t* cold_row = nullptr;  // real row of each second part
     ccd::SplitRow* srows = nullptr;  // per row: tile ranges of its two parts

@@ -103,6 +103,7 @@ __device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const Pas
     a.tile_gmax = nullptr;  // one tile_core copy here: every tile takes the general path
     a.cold_row = nullptr;   // the batched kernel works on plain layouts
     a.nreal = 0;
+    a.cold_tile0 = 0xffffffffu;
     a.part = nullptr;
     a.head = L.head;
     a.tail = L.tail;

@@ -330,6 +330,9 @@ struct TiledArgs {
     const uint32_t* cold_row;
     uint32_t nreal;
     void* part;  // (g, h) pairs of the accumulator type
+    // First tile of a split layout whose elements all belong to second parts
+    // (every gather misses the LDS prefix); 0xffffffff: none.
+    uint32_t cold_tile0;
 };
 
 // One row spanning tiles t0..t1 (t1 > t0) of a tiled layout: its row, first

@@ -157,7 +157,7 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
 // masked.  FLAT (plain sweeps of the batched kernel): one flat load per
 // gather instead of separate LDS and global loads.
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR, int SEG,
-          bool HOT = false, bool FLAT = false>
+          bool HOT = false, bool FLAT = false, bool COLD = false>
 __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, const FPair<TF>* ldsp, const uint32_t tile,
                                           const int lane, const uint32_t flags, const uint32_t seg0,
                                           const uint32_t row0, const uint32_t segoff, const uint32_t open,
@@ -222,6 +222,17 @@ __device__ __forceinline__ void tile_core(const TiledArgs& a, const TF* lds, con
             xsv[i] = 0.0f;
             xav[i] = 0.0f;
             xv[i] = gather<TF>(lds, a.hot_x, a.x, c[i]);
+        }
+    } else if constexpr (SWEEP && COLD) {
+        // every gather of the tile misses the staged prefix
+        TF xg[SIGMA];
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) xg[i] = ((glb_ptr<TF>)a.x)[c[i]];
+#pragma unroll
+        for (int i = 0; i < SIGMA; ++i) {
+            xsv[i] = 0.0f;
+            xav[i] = 0.0f;
+            xv[i] = tof<TF>(xg[i]);
         }
     } else if constexpr (SWEEP) {
         TF xg[SIGMA];
@@ -403,6 +414,17 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
     // common one has no segment-table loads (a load on only one of two merged
     // paths makes the compiler wait for every outstanding load at each later
     // use), no global gathers and, for 3-byte indices, no high bytes.
+    // Plain sweeps: a tile of second parts only (consecutive non-empty rows,
+    // no padding) gathers straight from global memory.
+    if constexpr (SPLIT && SWEEP && !SUB && !ADD) {
+        if (tile >= a.cold_tile0 && gmax != 0xffffffffu) {
+            load_tile<IdxT, SIGMA, false>(a, tile, lane, c, r, flags, seg0);
+            descriptors();
+            tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN, true, false, true>(
+                a, lds, ldsp, tile, lane, flags, seg0, row0, segoff, open, c, r);
+            return;
+        }
+    }
     if (SPLIT && SWEEP && gmax < hot_limit<IdxT, SUB, ADD, XCUR>(a)) {
         if constexpr (EARLY) descriptors();
         load_tile<IdxT, SIGMA, EARLY, false>(a, tile, lane, c, r, flags, seg0);
