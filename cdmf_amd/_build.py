@@ -40,6 +40,10 @@ HIP_SOURCES = [
     "batch.hip",
     "foldin.hip",
 ]
+# plain C++ parts of the HIP library (g++)
+LIB_CXX_SOURCES = [
+    "tile_layout.cpp",
+]
 HOST_SOURCES = [
     "host/tools.cpp",
     "host/cdmf_hip.cpp",
@@ -62,11 +66,30 @@ def build_data() -> str:
 
 
 def build_hip() -> str:
+    """One hipcc per source, in parallel (objects under _build/, next to the
+    library), then one link."""
     os.makedirs(LIB, exist_ok=True)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
     out = os.path.join(LIB, "libcdmf_hip.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + INC, "-I" + os.path.join(CSRC, "kernels"), *srcs,
-          "-o", out + ".tmp"])
+    inc = ["-I" + INC, "-I" + CSRC, "-I" + os.path.join(CSRC, "kernels")]
+    jobs, objs = [], []
+    for s in LIB_CXX_SOURCES:
+        obj = os.path.join(objdir, s.replace("/", "_").replace(".cpp", ".o"))
+        objs.append(obj)
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", *inc, "-c", os.path.join(CSRC, s), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        jobs.append((s, subprocess.Popen(cmd)))
+    for s in HIP_SOURCES:
+        obj = os.path.join(objdir, s.replace("/", "_").replace(".hip", ".o"))
+        objs.append(obj)
+        cmd = [HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, s), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        jobs.append((s, subprocess.Popen(cmd)))
+    failed = [s for s, p in jobs if p.wait() != 0]
+    if failed:
+        raise RuntimeError("hipcc failed on " + ", ".join(failed))
+    _run([HIPCC, *HIP_FLAGS, "-shared", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -65,6 +65,7 @@ def data_lib() -> C.CDLL:
         lib.cdmf_gen_powerlaw.argtypes = [C.POINTER(CGenSpec), C.POINTER(CDataset)]
         lib.cdmf_gen_uniform.argtypes = [C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32, C.c_uint64,
                                          C.POINTER(CDataset)]
+        lib.cdmf_gen_stress.argtypes = [C.c_char, C.c_uint64, C.POINTER(CDataset)]
         lib.cdmf_load_dir.argtypes = [C.c_char_p, C.POINTER(CDataset)]
         lib.cdmf_write_dir.argtypes = [C.c_char_p, C.POINTER(CDataset)]
         lib.cdmf_dataset_free.argtypes = [C.POINTER(CDataset)]

@@ -33,7 +33,7 @@ struct ccd_batch {
     // captured run
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<uint32_t> graph_key;
+    std::vector<uint64_t> graph_key;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     double last_seconds = 0.0;
     std::vector<uint32_t> team;  // blocks per problem of the last launch
@@ -449,11 +449,21 @@ int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph) {
     if (int rc = describe(b, sigma, blocks, sp, false, &a)) return rc;
     b->team = sp.nb;
 
-    // the captured sequence depends on the plan, the grid and each problem's
-    // continued / zero_first state
-    std::vector<uint32_t> key{(uint32_t)sigma, blocks};
-    for (const ccd_batch::Item& it : b->items)
+    // the captured sequence depends on the plan, the grid, each problem's
+    // continued / zero_first state and the buffers it was captured with:
+    // a context's generation moves whenever it reallocates any of them
+    // (upload, a new k, a rebuilt tile layout), and the addresses the
+    // capture bakes in are keyed as well
+    std::vector<uint64_t> key{(uint64_t)sigma, blocks};
+    for (const ccd_batch::Item& it : b->items) {
+        const ccd_ctx* c = it.ctx;
         key.push_back((it.ctx->continued ? 1u : 0u) | (it.p.zero_first ? 2u : 0u));
+        key.push_back(c->gen);
+        for (const void* q : {(const void*)c->W, (const void*)c->H, (const void*)c->u, (const void*)c->v,
+                              (const void*)c->csr.val, (const void*)c->csc.val, (const void*)c->csr.tl.tval,
+                              (const void*)c->csc.tl.tval, (const void*)c->csr.pair, (const void*)c->csc.pair})
+            key.push_back((uint64_t)(uintptr_t)q);
+    }
     BATCH_TRY(b, hipEventRecord(b->ev_a, b->stream));
     if (use_graph) {
         if (!b->exec || key != b->graph_key) {

@@ -361,7 +361,7 @@ int upload_vec(ccd_ctx* c, T** dst, const std::vector<T>& v) {
     return CCD_OK;
 }
 
-// Tile descriptors from the row pointer (host, O(rows + tiles)), index array
+// Tile descriptors from the row pointer (host, tile_layout.cpp), index array
 // tiled on the device.
 // split: with rows split at L.split_at, keep each row's two parts as two
 // virtual rows, all first parts (rows in order) before all second parts,
@@ -370,102 +370,31 @@ int upload_vec(ccd_ctx* c, T** dst, const std::vector<T>& v) {
 int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
     split = split && L.split_at != 0;
     if (L.tl.sigma == sigma && L.tl.split == split) return CCD_OK;
+    // a captured outer iteration (or batch) holds the old layout's pointers
+    drop_graph(c);
+    ++c->gen;
     free_tiled(L.tl);
     TiledLayout& T = L.tl;
-    const uint64_t TS = 64ull * (uint64_t)sigma, nnz = L.nnz;
-    const uint64_t ntiles = (nnz + TS - 1) / TS;
-    if (ntiles >= 0xffffffffull / 64) return set_err(c, CCD_ERR_INVALID, "too many tiles");
-    // virtual rows of a split layout: row pointer, the layout position of
-    // every element in tiled order, the real row of every second part and
-    // each row's parts and their tile ranges
-    std::vector<uint32_t> vptr, vpos, cold_row;
-    std::vector<ccd::SplitRow> srows;
-    if (split) {
-        const int64_t nr = L.rows;
-        const uint32_t* hp = L.hptr.data();
-        vptr.assign(1, 0u);
-        vptr.reserve((size_t)nr * 2 + 1);
-        srows.resize((size_t)nr);
-        for (int64_t r = 0; r < nr; ++r) vptr.push_back(vptr.back() + L.below[r]);
-        for (int64_t r = 0; r < nr; ++r) {
-            const uint32_t rest = hp[r + 1] - hp[r] - L.below[r];
-            ccd::SplitRow& q = srows[r];
-            q = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, hp[r + 1] - hp[r]};
-            if (L.below[r]) {
-                q.a0 = (uint32_t)(vptr[r] / TS);
-                q.a1 = (uint32_t)((vptr[r + 1] - 1ull) / TS);
-            }
-            if (rest) {
-                q.b = (uint32_t)(nr + cold_row.size());
-                q.b0 = (uint32_t)(vptr.back() / TS);
-                q.b1 = (uint32_t)((vptr.back() + rest - 1ull) / TS);
-                cold_row.push_back((uint32_t)r);
-                vptr.push_back(vptr.back() + rest);
-            }
-        }
-        vpos.resize(nnz);
-        for (int64_t r = 0; r < nr; ++r)
-            for (uint32_t q = 0; q < L.below[r]; ++q) vpos[vptr[r] + q] = hp[r] + q;
-        for (size_t k = 0; k < cold_row.size(); ++k) {
-            const uint32_t r = cold_row[k], b = vptr[nr + k];
-            for (uint32_t q = 0; q < vptr[nr + k + 1] - b; ++q) vpos[b + q] = hp[r] + L.below[r] + q;
-        }
-    }
-    const uint32_t* ptr = split ? vptr.data() : L.hptr.data();
-    const int64_t rows = split ? (int64_t)vptr.size() - 1 : L.rows;
-    std::vector<uint32_t> flags(ntiles * 64, 0u), row0(ntiles), segoff(ntiles), segrows;
-    std::vector<ccd::SpanRow> span;
-    std::vector<uint8_t> open((ntiles + 3) & ~(uint64_t)3);  // whole words: the kernels read it a word at a time
-    std::vector<uint32_t> emp(rows + 1, 0u);
-    for (int64_t r = 0; r < rows; ++r) {
-        const bool empty = ptr[r + 1] == ptr[r];
-        emp[r + 1] = emp[r] + (empty ? 1u : 0u);
-        if (empty) continue;
-        const uint64_t e = ptr[r], rem = e % TS;
-        flags[(e / TS) * 64 + rem / sigma] |= 1u << (rem % sigma);
-        const uint64_t t0 = e / TS, t1 = (ptr[r + 1] - 1ull) / TS;
-        if (t0 != t1) span.push_back({(uint32_t)r, (uint32_t)t0, (uint32_t)t1, ptr[r + 1] - ptr[r]});
-    }
-    int64_t r = 0;
-    for (uint64_t t = 0; t < ntiles; ++t) {
-        const uint64_t tbase = t * TS;
-        while ((uint64_t)ptr[r + 1] <= tbase) ++r;
-        row0[t] = (uint32_t)r;
-        uint8_t o = (uint64_t)ptr[r] < tbase ? 1 : 0;
-        const uint64_t tend = std::min<uint64_t>(tbase + TS, nnz) - 1;
-        int64_t r2 = r;
-        while ((uint64_t)ptr[r2 + 1] <= tend) ++r2;
-        if ((uint64_t)ptr[r2 + 1] > tbase + TS) o |= 2;
-        open[t] = o;
-        if (r2 > r + 1 && emp[r2] - emp[r + 1] > 0) {
-            segoff[t] = (uint32_t)segrows.size();
-            for (int64_t q = r; q <= r2; ++q)
-                if (ptr[q + 1] > ptr[q]) segrows.push_back((uint32_t)q);
-        } else {
-            segoff[t] = 0xffffffffu;
-        }
-    }
-    T.sigma = sigma;
-    T.split = split;
+    ccd::TileTables H;
+    const std::string why =
+        ccd::build_tile_tables(L.hptr.data(), L.rows, L.nnz, split ? L.below.data() : nullptr, sigma, split, H);
+    if (!why.empty()) return set_err(c, CCD_ERR_INVALID, "tile layout: " + why);
+    const uint64_t TS = 64ull * (uint64_t)sigma, nnz = L.nnz, ntiles = H.ntiles;
     T.ntiles = (uint32_t)ntiles;
-    // two-tile spans first (finished one per lane), the rest after them
-    std::stable_partition(span.begin(), span.end(), [](const ccd::SpanRow& q) { return q.t1 == q.t0 + 1; });
-    T.nspan = (uint32_t)span.size();
-    T.nshort = (uint32_t)std::count_if(span.begin(), span.end(), [](const ccd::SpanRow& q) { return q.t1 == q.t0 + 1; });
+    T.nspan = (uint32_t)H.spans.size();
+    T.nshort = H.nshort;
     int rc;
-    if ((rc = upload_vec(c, &T.flags, flags)) ||
-        (rc = upload_vec(c, &T.row0, row0)) || (rc = upload_vec(c, &T.segoff, segoff)) ||
-        (rc = upload_vec(c, &T.segrows, segrows)) || (rc = upload_vec(c, &T.open, open)) ||
-        (rc = upload_vec(c, &T.spans, span)))
+    if ((rc = upload_vec(c, &T.flags, H.flags)) || (rc = upload_vec(c, &T.row0, H.row0)) ||
+        (rc = upload_vec(c, &T.segoff, H.segoff)) || (rc = upload_vec(c, &T.segrows, H.segrows)) ||
+        (rc = upload_vec(c, &T.open, H.open)) || (rc = upload_vec(c, &T.spans, H.spans)))
         return rc;
-    T.cold_tile0 = 0xffffffffu;
+    T.cold_tile0 = H.cold_tile0;
     if (split) {
-        T.nreal = (uint32_t)L.rows;
-        T.cold_tile0 = (uint32_t)((vptr[L.rows] + TS - 1) / TS);
-        if ((rc = upload_vec(c, &T.vpos, vpos)) || (rc = upload_vec(c, &T.cold_row, cold_row)) ||
-            (rc = upload_vec(c, &T.srows, srows)))
+        T.nreal = H.nreal;
+        if ((rc = upload_vec(c, &T.vpos, H.vpos)) || (rc = upload_vec(c, &T.cold_row, H.cold_row)) ||
+            (rc = upload_vec(c, &T.srows, H.srows)))
             return rc;
-        HIP_TRY(c, dalloc(&T.part, (size_t)rows));
+        HIP_TRY(c, dalloc(&T.part, H.vptr.size() - 1));
     }
     const uint64_t padded = ntiles * TS;
     HIP_TRY(c, dalloc(&T.tval, padded));
@@ -483,6 +412,8 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
         HIP_TRY(c, ccd::launch_tile_gmax(L.idx, L.idx16, T.segoff, T.gmax, nnz, T.ntiles, sigma, T.vpos, c->stream));
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    T.sigma = sigma;  // only a complete layout is reused
+    T.split = split;
     return CCD_OK;
 }
 
@@ -795,6 +726,7 @@ int32_t ccd_set_factors(ccd_ctx* c, uint32_t k, const float* W, const float* H) 
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (k != c->k) {
         drop_graph(c);
+        ++c->gen;
         dfree(c->W);
         dfree(c->H);
         c->W = c->H = nullptr;

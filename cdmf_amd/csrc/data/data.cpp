@@ -352,6 +352,78 @@ int cdmf_gen_uniform(int64_t m, int64_t n, uint64_t nnz_train, uint64_t nnz_test
     return assemble(m, n, row_off, cols, vals, is_test, out);
 }
 
+int cdmf_gen_stress(char shape, uint64_t seed, cdmf_dataset* out) {
+    if (!out) return fail(CDMF_ERR_INVALID, "null output");
+    int64_t m, n, dense_row;
+    uint32_t fixed_col, lo, hi;  // every sparse row: fixed_col and one column drawn from [lo, hi)
+    if (shape == 'a') {
+        m = 64000, n = 150, dense_row = 0, fixed_col = 0, lo = 1, hi = 150;
+    } else if (shape == 'b') {
+        m = n = 1024000, dense_row = m - 1, fixed_col = (uint32_t)(n - 1), lo = 0, hi = (uint32_t)(n - 1);
+    } else if (shape == 'c') {
+        m = n = 1024000, dense_row = -1, fixed_col = 0, lo = 1, hi = (uint32_t)n;
+    } else {
+        return fail(CDMF_ERR_INVALID, "stress shape must be 'a', 'b' or 'c'");
+    }
+    RatingTable rt(CDMF_RATING_UNIFORM_1_5);
+    // training ratings: the drawn column of each sparse row, one stream per row
+    std::vector<uint32_t> drawn(m, 0);
+    std::vector<float> r_fixed(m), r_drawn(m);
+#pragma omp parallel for schedule(static, 4096)
+    for (int64_t i = 0; i < m; ++i) {
+        Xoshiro g(stream_seed(seed, 0x5a, (uint64_t)i));
+        drawn[i] = lo + (uint32_t)g.below(hi - lo);
+        r_fixed[i] = rt.draw(g);
+        r_drawn[i] = rt.draw(g);
+    }
+    const uint64_t ntrain = (dense_row >= 0 ? (uint64_t)n + 2ull * (uint64_t)(m - 1) : 2ull * (uint64_t)m);
+    const uint64_t ntest = ntrain / 100;
+    // test positions: sparse rows only, a column of [lo, hi) other than the
+    // row's drawn one (so apart from the row's training ratings), distinct
+    std::vector<std::vector<std::pair<uint32_t, float>>> test(m);
+    {
+        Xoshiro g(stream_seed(seed, 0x7e5, 0));
+        std::unordered_set<uint64_t> seen;
+        uint64_t got = 0;
+        while (got < ntest) {
+            const int64_t i = (int64_t)g.below((uint64_t)m);
+            if (i == dense_row) continue;
+            const uint32_t j = lo + (uint32_t)g.below(hi - lo);
+            if (j == drawn[i] || j == fixed_col) continue;
+            if (!seen.insert((uint64_t)i * (uint64_t)n + j).second) continue;
+            test[i].push_back({j, rt.draw(g)});
+            ++got;
+        }
+    }
+    std::vector<uint64_t> row_off(m + 1, 0);
+    for (int64_t i = 0; i < m; ++i)
+        row_off[i + 1] = row_off[i] + (i == dense_row ? (uint64_t)n : 2u) + test[i].size();
+    const uint64_t total = row_off[m];
+    std::vector<uint32_t> cols(total);
+    std::vector<float> vals(total);
+    std::vector<uint8_t> is_test(total, 0);
+    Xoshiro gd(stream_seed(seed, 0xde5e, 0));
+    for (int64_t i = 0; i < m; ++i) {
+        std::vector<std::pair<uint32_t, std::pair<float, uint8_t>>> row;
+        if (i == dense_row) {
+            for (int64_t j = 0; j < n; ++j) row.push_back({(uint32_t)j, {rt.draw(gd), 0}});
+        } else {
+            row.push_back({fixed_col, {r_fixed[i], 0}});
+            row.push_back({drawn[i], {r_drawn[i], 0}});
+        }
+        for (const auto& t : test[i]) row.push_back({t.first, {t.second, 1}});
+        std::sort(row.begin(), row.end());
+        uint64_t e = row_off[i];
+        for (const auto& q : row) {
+            cols[e] = q.first;
+            vals[e] = q.second.first;
+            is_test[e] = q.second.second;
+            ++e;
+        }
+    }
+    return assemble(m, n, row_off, cols, vals, is_test, out);
+}
+
 int cdmf_gen_powerlaw(const cdmf_gen_spec* spec, cdmf_dataset* out) {
     if (!spec || !out || spec->m <= 0 || spec->n <= 0) return fail(CDMF_ERR_INVALID, "bad shape");
     const int64_t m = spec->m, n = spec->n;

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tile_layout.h"  // SpanRow, SplitRow
+
 #include <type_traits>
 
 namespace ccd {
@@ -335,12 +337,6 @@ struct TiledArgs {
     uint32_t cold_tile0;
 };
 
-// One row spanning tiles t0..t1 (t1 > t0) of a tiled layout: its row, first
-// and last tile and length.
-struct SpanRow {
-    uint32_t row, t0, t1, len;
-};
-
 // spans[0, nshort) cover two tiles (t1 = t0 + 1) and are finished one per
 // lane; spans[nshort, nspan) cover more and are finished one per wave.
 struct SpanArgs {
@@ -352,13 +348,6 @@ struct SpanArgs {
     uint32_t nshort;
     float lambda;
     void* part;  // split layout: (g, h) of each spanning virtual row go here instead of finishing it
-};
-
-// Row r of a split layout: its first part (virtual row r) over tiles a0..a1
-// (a0 = 0xffffffff: empty), its second part (virtual row b, 0xffffffff:
-// none) over tiles b0..b1, and its length.
-struct SplitRow {
-    uint32_t a0, a1, b, b0, b1, len;
 };
 
 // Finish of a split layout's sweep, one wave per row: each part's (g, h) is

@@ -35,7 +35,12 @@ SMALL_SOLVE = SolveConfig(k=10, lam=0.05, outer=5, inner=5)
 # The large case: 480,189 x 17,770 with 99,072,112 train + 1,408,395 test
 # ratings, power-law row degrees (min 1, max 17,000), column popularity
 # rank^-1.0, ratings 1..5 with P = [.05 .10 .29 .34 .22]; k=40, lambda=0.05,
-# 10 outer x 5 inner.
+# 10 outer x 5 inner.  The row-degree exponent is fitted (row_alpha = 0) so
+# that the law on [1, 17,000] has the matrix's mean degree: about 1.38
+# (row_degree_exponent).  A Zipf exponent of 1.2 on the same range would
+# give a mean of about 620 ratings per user, three times the target;
+# rescaling its draws to the total would cut the largest degrees to about
+# 5,800.
 NETFLIX = dict(m=480189, n=17770, nnz_train=99072112, nnz_test=1408395, max_row_degree=17000,
                row_alpha=0.0, col_exponent=1.0, rating_kind=1, seed=1)
 NETFLIX_SOLVE = SolveConfig(k=40, lam=0.05, outer=10, inner=5)
@@ -109,6 +114,22 @@ def gen_uniform(m: int, n: int, nnz_train: int, nnz_test: int, seed: int = 0,
     return N.dataset_from_c(cd)
 
 
+# The reference's CTest stress shapes and their flags (CMakeLists.txt:61-63:
+# -k 10 -t 5 -T 3 -l 0.05).
+STRESS_SHAPES = ("a", "b", "c")
+STRESS_SOLVE = SolveConfig(k=10, lam=0.05, outer=5, inner=3)
+
+
+def gen_stress(shape: str, seed: int = 0) -> Dataset:
+    """data/{a,b,c}/{a,b,c}.py of the reference, seeded (cdmf_gen_stress):
+    a dense first row (a), a dense last row and last column (b) or a dense
+    first column (c), every other row holding two ratings; the test set
+    (train / 100 entries) lies apart from the training ratings."""
+    cd = N.CDataset()
+    N.check_data(N.data_lib().cdmf_gen_stress(shape.encode(), seed, C.byref(cd)))
+    return N.dataset_from_c(cd)
+
+
 def gen_powerlaw(m: int, n: int, nnz_train: int, nnz_test: int, max_row_degree: int = 0,
                  row_alpha: float = 0.0, col_exponent: float = 1.0, rating_kind: int = 1,
                  seed: int = 0, threads: int = 0) -> Dataset:
@@ -118,6 +139,34 @@ def gen_powerlaw(m: int, n: int, nnz_train: int, nnz_test: int, max_row_degree: 
     cd = N.CDataset()
     N.check_data(N.data_lib().cdmf_gen_powerlaw(C.byref(spec), C.byref(cd)))
     return N.dataset_from_c(cd)
+
+
+def row_degree_exponent(m: int, nnz_train: int, nnz_test: int, max_row_degree: int, n: int,
+                        row_alpha: float = 0.0, **_) -> float:
+    """The row-degree exponent the power-law generator uses for a spec: the
+    given row_alpha, or, for row_alpha = 0, the exponent a whose law
+    P(d) ~ d^-a on [1, max_row_degree] has mean degree (train + test) / m
+    (the bisection of cdmf_gen_powerlaw, restated)."""
+    if row_alpha > 0.0:
+        return row_alpha
+    D = min(max_row_degree or n, n)
+    d = np.arange(1, D + 1, dtype=np.float64)
+    target = (nnz_train + nnz_test) / m
+
+    def mean(a):
+        w = d ** -a
+        return float((w * d).sum() / w.sum())
+
+    if mean(0.0) < target:
+        return 0.0
+    lo, hi = 0.0, 6.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if mean(mid) > target:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
 
 
 def small() -> Dataset:

@@ -88,6 +88,20 @@ int cdmf_gen_powerlaw(const cdmf_gen_spec* spec, cdmf_dataset* out);
 int cdmf_gen_uniform(int64_t m, int64_t n, uint64_t nnz_train, uint64_t nnz_test, int32_t rating_kind,
                      uint64_t seed, cdmf_dataset* out);
 
+/* The reference's CTest stress shapes (data/a/a.py:25-55, data/b/b.py:24-55,
+ * data/c/c.py:25-43), seeded, with ratings uniform in 1..5:
+ *   'a': 64,000 x 150; row 0 holds every column, every other row column 0
+ *        and one column drawn from [1, n)            (128,148 train ratings)
+ *   'b': 1,024,000 x 1,024,000; every row but the last holds one column
+ *        drawn from [0, n-1) and column n-1, the last row every column
+ *                                                    (3,071,998 train ratings)
+ *   'c': 1,024,000 x 1,024,000; every row holds column 0 and one column
+ *        drawn from [1, n)                           (2,048,000 train ratings)
+ * The test set has train/100 entries, like the reference's, but at
+ * positions drawn apart from the training ratings (the reference takes its
+ * test lines from the head of the training file, so they overlap). */
+int cdmf_gen_stress(char shape, uint64_t seed, cdmf_dataset* out);
+
 /* Read a data directory (meta_modified_all + binaries).  A compressed
  * pointer file shorter than m+1 / n+1 entries is padded with nnz.  Errors:
  * CDMF_ERR_IO with a message available from cdmf_data_last_error(). */

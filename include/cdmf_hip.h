@@ -211,9 +211,11 @@ int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);
  * zero_first; FP32 only, maxinneriter >= 1).  The batch borrows the
  * contexts: keep them alive and do not use them while the batch runs.
  * After ccd_batch_run every context holds the result of its own ccd_run
- * with the tiled sweeps (bit-identical, same sigma): read it with
- * ccd_get_factors / ccd_get_residual / ccd_rmse, continue it with another
- * batch run or a ccd_run.
+ * with the tiled sweeps on the plain (unsplit) tile layout, i.e. with
+ * ccd_plan_params.split_rows = 0 (bit-identical, same sigma; a ccd_run with
+ * split_rows = 1 rounds differently wherever a gathered vector exceeds the
+ * LDS prefix): read it with ccd_get_factors / ccd_get_residual / ccd_rmse,
+ * continue it with another batch run or a ccd_run.
  */
 typedef struct ccd_batch ccd_batch;
 
@@ -237,8 +239,9 @@ int32_t ccd_batch_get_plan(ccd_batch* b, ccd_batch_plan* out);
 int32_t ccd_batch_tune(ccd_batch* b, double* best_ms);
 /* One run of every problem: set-up fills and retiling, the persistent
  * launch, write-back.  use_graph: capture that sequence in a HIP graph on
- * first use and replay it while the plan and the problems' continued /
- * zero_first state stay the same. */
+ * first use and replay it while the plan, the problems' continued /
+ * zero_first state and their device buffers stay the same (a re-upload, a
+ * new k or a ccd_run that rebuilds a tile layout recaptures it). */
 int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph);
 /* device time of the last run (events around the whole sequence) */
 int32_t ccd_batch_time(ccd_batch* b, double* seconds);

@@ -7,8 +7,11 @@ cpu_baseline leg may import this package, and only as the checker.
 Function -> reference it restates (see ccd_oracle.c for the details):
   ccd        src/cdmf_ref.cpp:47-140     rmse        src/tools.cpp:543-582
   initial_col src/tools.cpp:349-357      dual_spmv   src/detail/kcode/csr5_spmv.cl:134-147
-  ccd_f64 / ccd_bf16 / objective / residual_f64 / sweep_f64: no reference
-  counterpart; pinned by the mathematical checks in tests/test_oracle.py.
+  ccd_ranks  slices of ccd (the CPU baseline); chained, equal to ccd bit for bit
+  ccd_f64 / ccd_bf16 / objective / residual_f64 / sweep_f64 / train RMSE:
+  no reference counterpart; pinned by the checks in tests/test_oracle.py
+  (an independent numpy restatement of the BF16 twin, the twin built with
+  identity rounding against ccd, mathematics for the rest).
   fold_rows / fold_features / mlp_forward / mlp_backward (foldin_oracle.c):
   the fold-in extension, no reference counterpart; pinned by
   tests/test_oracle_foldin.py (ridge solve, finite differences).
@@ -23,9 +26,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libccd_oracle.so")
+LIB_BF16ID = os.path.join(HERE, "_build", "libccd_oracle_bf16id.so")
 REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
 
 _lib = None
+_lib_bf16id = None
 
 u32p = C.POINTER(C.c_uint32)
 f32p = C.POINTER(C.c_float)
@@ -36,6 +41,27 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
 
 
+def _declare(L: C.CDLL) -> None:
+    i64, u64, u32, i32, flt, dbl = C.c_int64, C.c_uint64, C.c_uint32, C.c_int, C.c_float, C.c_double
+    L.or_ccd.argtypes = [i64, i64, u32p, u32p, f32p, u32p, u32p, f32p, f32p, f32p, u32, flt, i32, i32, i32, i32,
+                         i32, u64, u32p, u32p, f32p, f64p, f64p]
+    L.or_ccd_bf16.argtypes = L.or_ccd.argtypes
+    L.or_ccd_f64.argtypes = [i64, i64, u32p, u32p, f64p, u32p, u32p, f64p, f64p, f64p, u32, dbl, i32, i32, i32,
+                             i32, i32, u64, u32p, u32p, f32p, f64p, f64p]
+
+
+def lib_bf16_identity() -> C.CDLL:
+    """The oracle built with the BF16 twin's rounding replaced by the identity."""
+    global _lib_bf16id
+    if _lib_bf16id is None:
+        if not os.path.exists(LIB_BF16ID):
+            build()
+        L = C.CDLL(LIB_BF16ID)
+        _declare(L)
+        _lib_bf16id = L
+    return _lib_bf16id
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -43,11 +69,7 @@ def lib() -> C.CDLL:
             build()
         L = C.CDLL(LIB)
         i64, u64, u32, i32, flt, dbl = C.c_int64, C.c_uint64, C.c_uint32, C.c_int, C.c_float, C.c_double
-        L.or_ccd.argtypes = [i64, i64, u32p, u32p, f32p, u32p, u32p, f32p, f32p, f32p, u32, flt, i32, i32, i32, i32,
-                             i32, u64, u32p, u32p, f32p, f64p]
-        L.or_ccd_bf16.argtypes = L.or_ccd.argtypes
-        L.or_ccd_f64.argtypes = [i64, i64, u32p, u32p, f64p, u32p, u32p, f64p, f64p, f64p, u32, dbl, i32, i32, i32,
-                                 i32, i32, u64, u32p, u32p, f32p, f64p]
+        _declare(L)
         L.or_ccd_ranks.argtypes = [i64, i64, u32p, u32p, f32p, u32p, u32p, f32p, f32p, f32p, u32, u32, flt, i32, i32,
                                    i32]
         L.or_rmse.argtypes = [f32p, f32p, i64, i64, u32, u64, u32p, u32p, f32p]
@@ -86,12 +108,18 @@ def _c(a, dtype):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
-def ccd(ds, W, H, k, lam, outer, inner, order=0, zero_first=True, threads=0, dtype="f32"):
+def ccd(ds, W, H, k, lam, outer, inner, order=0, zero_first=True, threads=0, dtype="f32", train_out=None,
+        _lib=None):
     """Run the restated cdmf_ref on copies of ds's values and of W, H.
 
     Returns (W, H, rmse_per_iter, csr_residual, csc_residual).  dtype is
-    "f32" (the reference), "bf16" (BF16-stored factors) or "f64"."""
-    L = lib()
+    "f32" (the reference), "bf16" (BF16-stored factors) or "f64".
+    train_out: a float64 array of at least `outer` entries that receives the
+    train RMSE after every outer iteration."""
+    L = _lib or lib()
+    if train_out is not None:
+        assert train_out.dtype == np.float64 and train_out.flags.c_contiguous and train_out.size >= outer
+    tp = train_out.ctypes.data_as(f64p) if train_out is not None else None
     cp, ci, cq, ki = _c(ds.csr_ptr, np.uint32), _c(ds.csr_idx, np.uint32), _c(ds.csc_ptr, np.uint32), _c(ds.csc_idx, np.uint32)
     tr, tc, tv = _c(ds.test_row, np.uint32), _c(ds.test_col, np.uint32), _c(ds.test_val, np.float32)
     rm = np.full(max(outer, 1), np.nan)
@@ -101,7 +129,7 @@ def ccd(ds, W, H, k, lam, outer, inner, order=0, zero_first=True, threads=0, dty
         rc = L.or_ccd_f64(ds.m, ds.n, _p(cp, C.c_uint32), _p(ci, C.c_uint32), _p(rv, C.c_double),
                           _p(cq, C.c_uint32), _p(ki, C.c_uint32), _p(kv, C.c_double), _p(Wc, C.c_double),
                           _p(Hc, C.c_double), k, lam, outer, inner, order, int(zero_first), threads, tr.size,
-                          _p(tr, C.c_uint32), _p(tc, C.c_uint32), _p(tv, C.c_float), _p(rm, C.c_double))
+                          _p(tr, C.c_uint32), _p(tc, C.c_uint32), _p(tv, C.c_float), _p(rm, C.c_double), tp)
     else:
         rv, kv = ds.csr_val.astype(np.float32).copy(), ds.csc_val.astype(np.float32).copy()
         Wc, Hc = np.array(W, dtype=np.float32), np.array(H, dtype=np.float32)
@@ -109,24 +137,36 @@ def ccd(ds, W, H, k, lam, outer, inner, order=0, zero_first=True, threads=0, dty
         rc = f(ds.m, ds.n, _p(cp, C.c_uint32), _p(ci, C.c_uint32), _p(rv, C.c_float),
                _p(cq, C.c_uint32), _p(ki, C.c_uint32), _p(kv, C.c_float), _p(Wc, C.c_float), _p(Hc, C.c_float),
                k, lam, outer, inner, order, int(zero_first), threads, tr.size,
-               _p(tr, C.c_uint32), _p(tc, C.c_uint32), _p(tv, C.c_float), _p(rm, C.c_double))
+               _p(tr, C.c_uint32), _p(tc, C.c_uint32), _p(tv, C.c_float), _p(rm, C.c_double), tp)
     if rc != 0:
         raise RuntimeError("oracle failed (%d)" % rc)
     return Wc, Hc, rm[:outer], rv, kv
 
 
-def ccd_ranks(ds, W, H, t0, t1, lam, oiter, inner, threads=0):
-    """Ranks [t0, t1) of outer iteration `oiter`, in place on copies; returns (W, H)."""
+class RankState:
+    """Arrays of an H-first solve advanced slice by slice with ccd_ranks:
+    contiguous copies of the index arrays, the two residual copies (R at
+    the start) and W, H, all updated in place.  Building it copies; the
+    rank slices do not."""
+
+    def __init__(self, ds, W, H):
+        self.m, self.n = ds.m, ds.n
+        self.cp, self.ci, self.cq, self.ki = (_c(a, np.uint32) for a in (ds.csr_ptr, ds.csr_idx, ds.csc_ptr,
+                                                                        ds.csc_idx))
+        self.rv, self.kv = ds.csr_val.astype(np.float32).copy(), ds.csc_val.astype(np.float32).copy()
+        self.W, self.H = np.array(W, dtype=np.float32), np.array(H, dtype=np.float32)
+
+
+def ccd_ranks(st: RankState, t0, t1, lam, oiter, inner, threads=0) -> None:
+    """Ranks [t0, t1) of outer iteration `oiter` of an H-first solve, in
+    place on `st` (the caller zeroes H before outer iteration 1, as or_ccd
+    does at entry)."""
     L = lib()
-    cp, ci, cq, ki = (_c(a, np.uint32) for a in (ds.csr_ptr, ds.csr_idx, ds.csc_ptr, ds.csc_idx))
-    rv, kv = ds.csr_val.astype(np.float32).copy(), ds.csc_val.astype(np.float32).copy()
-    Wc, Hc = np.array(W, dtype=np.float32), np.array(H, dtype=np.float32)
-    rc = L.or_ccd_ranks(ds.m, ds.n, _p(cp, C.c_uint32), _p(ci, C.c_uint32), _p(rv, C.c_float), _p(cq, C.c_uint32),
-                        _p(ki, C.c_uint32), _p(kv, C.c_float), _p(Wc, C.c_float), _p(Hc, C.c_float), t0, t1, lam,
-                        oiter, inner, threads)
+    rc = L.or_ccd_ranks(st.m, st.n, _p(st.cp, C.c_uint32), _p(st.ci, C.c_uint32), _p(st.rv, C.c_float),
+                        _p(st.cq, C.c_uint32), _p(st.ki, C.c_uint32), _p(st.kv, C.c_float), _p(st.W, C.c_float),
+                        _p(st.H, C.c_float), t0, t1, lam, oiter, inner, threads)
     if rc != 0:
         raise RuntimeError("oracle failed (%d)" % rc)
-    return Wc, Hc
 
 
 def rmse(W, H, ds) -> float:

@@ -18,12 +18,24 @@
  *   or_ccd_f64           FP64 twin of or_ccd (precision study; no reference counterpart)
  *   or_ccd_bf16          or_ccd with every stored W/H value rounded to bfloat16 (no reference counterpart)
  *   or_objective         the objective CCD++ minimises, in FP64 (cdmf_ref.cpp:27,38 computes sum r^2 and drops it)
+ *   or_ccd_ranks         ranks [t0, t1) of one outer iteration of or_ccd, in place (CPU baseline)
+ *
+ * Train RMSE: every solver here can also report, per outer iteration, the
+ * training RMSE sqrt(sum rhat^2 / nnz) of the residual R - W H^T left after
+ * that iteration, summed in double in CSR order.  The reference computes
+ * sum rhat^2 inside UpdateRating (cdmf_ref.cpp:27,38) and discards it; this
+ * is that quantity for the whole matrix, taken where each outer iteration
+ * ends.
  *
  * Pinning: or_ccd / or_rmse / or_initial_col are checked bit-for-bit against
  * fixtures written by the compiled reference (oracle/Makefile, tests/golden/).
- * or_ccd_f64, or_ccd_bf16, or_dual_spmv and or_objective are pinned by
- * mathematical checks in tests/test_oracle.py (closed-form ridge solves, a
- * monotone objective, the transpose symmetry, scipy SpMV).
+ * or_ccd_ranks equals or_ccd bit-for-bit when its rank slices are chained.
+ * or_ccd_bf16 equals an independent numpy restatement with the same rounding
+ * points bit-for-bit, and, built with OR_BF16_IDENTITY (rounding replaced
+ * by the identity), equals or_ccd bit-for-bit.  or_ccd_f64, or_dual_spmv,
+ * or_objective and the train RMSE are pinned by mathematical checks in
+ * tests/test_oracle.py (closed-form ridge solves, a monotone objective, the
+ * transpose symmetry, scipy SpMV, a numpy recomputation).
  *
  * Build with -ffp-contract=off so that products and sums round exactly as in
  * the reference build (plain x86-64 SSE, no FMA).
@@ -97,6 +109,20 @@ OR_EXPORT double or_rmse(const float* W, const float* H, int64_t m, int64_t n, u
     return sqrt(rmse / (double)num);
 }
 
+/* Train RMSE of a residual array: sqrt(sum r^2 / nnz), summed in double in
+ * array order; NaN for an empty matrix. */
+static double or_train_rmse(const float* res, uint64_t nnz) {
+    double s = 0.0;
+    for (uint64_t e = 0; e < nnz; ++e) s += (double)res[e] * (double)res[e];
+    return nnz ? sqrt(s / (double)nnz) : NAN;
+}
+
+static double or_train_rmse_f64(const double* res, uint64_t nnz) {
+    double s = 0.0;
+    for (uint64_t e = 0; e < nnz; ++e) s += res[e] * res[e];
+    return nnz ? sqrt(s / (double)nnz) : NAN;
+}
+
 /* tools.cpp:349-357 -- X[j][i] = 0.1*(rand()/RAND_MAX) + 0.001 with srand(0)
  * before the draws, i-major order.  X is [k][n] rank-major. */
 OR_EXPORT void or_initial_col(float* X, uint32_t k, uint32_t n) {
@@ -120,7 +146,8 @@ OR_EXPORT void or_initial_col(float* X, uint32_t k, uint32_t n) {
  *              (cdmf_ref.cpp:57-61); 0 keeps the caller's values (the caller
  *              then guarantees that their product is zero, e.g. W = 0).
  * R's value arrays are updated in place and hold the residual on return,
- * as in the reference.  rmse_out (may be NULL) receives maxiter test RMSEs.
+ * as in the reference.  rmse_out and train_out (either may be NULL) receive
+ * maxiter test and train RMSEs.
  */
 OR_EXPORT int or_ccd(int64_t m, int64_t n,
                      const uint32_t* csr_ptr, const uint32_t* csr_idx, float* csr_val,
@@ -128,7 +155,7 @@ OR_EXPORT int or_ccd(int64_t m, int64_t n,
                      float* W, float* H, uint32_t k, float lambda, int maxiter, int maxinner,
                      int order, int zero_first, int threads,
                      uint64_t nnz_test, const uint32_t* trow, const uint32_t* tcol, const float* tval,
-                     double* rmse_out) {
+                     double* rmse_out, double* train_out) {
     /* "first" layout F is swept first and produces factor XF; "second" layout S
      * produces XS.  order 0: F = CSC (dim n, gathers W), S = CSR (dim m). */
     const int64_t dF = order ? m : n, dS = order ? n : m;
@@ -179,6 +206,7 @@ OR_EXPORT int or_ccd(int64_t m, int64_t n,
             or_update_rating(dS, pS, iS, vS, v, u, 0);
         }
         if (rmse_out) rmse_out[oiter - 1] = or_rmse(W, H, m, n, k, nnz_test, trow, tcol, tval);
+        if (train_out) train_out[oiter - 1] = or_train_rmse(csr_val, csr_ptr[m]);
     }
     free(u);
     free(v);
@@ -237,6 +265,11 @@ OR_EXPORT int or_ccd_ranks(int64_t m, int64_t n,
 /* nearest bfloat16 (ties to even); arithmetic stays FP32.                    */
 /* ---------------------------------------------------------------------- */
 
+#ifdef OR_BF16_IDENTITY
+/* Check build (oracle/Makefile, libccd_oracle_bf16id.so): no rounding, so
+ * or_ccd_bf16 must reproduce or_ccd bit for bit. */
+static inline float or_bf16_round(float x) { return x; }
+#else
 static inline float or_bf16_round(float x) {
     uint32_t b;
     memcpy(&b, &x, 4);
@@ -252,6 +285,7 @@ static inline float or_bf16_round(float x) {
     memcpy(&y, &b, 4);
     return y;
 }
+#endif
 
 OR_EXPORT float or_bf16(float x) { return or_bf16_round(x); }
 
@@ -261,7 +295,7 @@ OR_EXPORT int or_ccd_bf16(int64_t m, int64_t n,
                           float* W, float* H, uint32_t k, float lambda, int maxiter, int maxinner,
                           int order, int zero_first, int threads,
                           uint64_t nnz_test, const uint32_t* trow, const uint32_t* tcol, const float* tval,
-                          double* rmse_out) {
+                          double* rmse_out, double* train_out) {
     const int64_t dF = order ? m : n, dS = order ? n : m;
     const uint32_t* pF = order ? csr_ptr : csc_ptr;
     const uint32_t* iF = order ? csr_idx : csc_idx;
@@ -304,6 +338,7 @@ OR_EXPORT int or_ccd_bf16(int64_t m, int64_t n,
             or_update_rating(dS, pS, iS, vS, v, u, 0);
         }
         if (rmse_out) rmse_out[oiter - 1] = or_rmse(W, H, m, n, k, nnz_test, trow, tcol, tval);
+        if (train_out) train_out[oiter - 1] = or_train_rmse(csr_val, csr_ptr[m]);
     }
     free(u);
     free(v);
@@ -359,7 +394,7 @@ OR_EXPORT int or_ccd_f64(int64_t m, int64_t n,
                          double* W, double* H, uint32_t k, double lambda, int maxiter, int maxinner,
                          int order, int zero_first, int threads,
                          uint64_t nnz_test, const uint32_t* trow, const uint32_t* tcol, const float* tval,
-                         double* rmse_out) {
+                         double* rmse_out, double* train_out) {
     const int64_t dF = order ? m : n, dS = order ? n : m;
     const uint32_t* pF = order ? csr_ptr : csc_ptr;
     const uint32_t* iF = order ? csr_idx : csc_idx;
@@ -399,6 +434,7 @@ OR_EXPORT int or_ccd_f64(int64_t m, int64_t n,
             or_update_rating_f64(dS, pS, iS, vS, v, u, 0);
         }
         if (rmse_out) rmse_out[oiter - 1] = or_rmse_f64(W, H, m, n, k, nnz_test, trow, tcol, tval);
+        if (train_out) train_out[oiter - 1] = or_train_rmse_f64(csr_val, csr_ptr[m]);
     }
     free(u);
     free(v);

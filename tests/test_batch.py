@@ -178,3 +178,31 @@ def test_batch_tuned_plan_and_oracle(oracle_lib):
         assert frob <= 1e-3, "%s vs oracle: %.3g" % (what, frob)
     assert abs(solvers[0].rmse() - rm[-1]) <= 1e-4
     b.close()
+
+
+def test_batch_graph_after_layout_rebuild():
+    """A captured batch run bakes in each context's buffer addresses.  A
+    ccd_run with split rows in between rebuilds the tile layout of a problem
+    whose gathered dimension (70,000 rows) exceeds the LDS prefix, freeing
+    the plain layout the capture used; the next graph run must recapture.
+    The same sequence with eager launches is the reference, bit for bit."""
+    S = _solver()
+    cases = _cases()[:3]
+    out = {}
+    for graph in (False, True):
+        b, solvers = _build(S, cases, 16)
+        _say("graph=%s: batch run 1" % graph)
+        b.run(use_graph=graph)
+        name, ds, k, lam, outer, inner, order, W0, H0 = cases[0]
+        _say("graph=%s: ccd_run with split rows on %s" % (graph, name))
+        solvers[0].set_plan(split_rows=1)
+        solvers[0].run(k, lam, 1, inner, variant=S.VARIANT_TILED, order=order, zero_first=False)
+        _say("graph=%s: batch run 2" % graph)
+        b.run(use_graph=graph)
+        out[graph] = _batch_results(solvers)
+        b.close()
+        for s in solvers:
+            s.close()
+    for (name, *_), e, g in zip(cases, out[False], out[True]):
+        assert all(np.all(np.isfinite(x)) for x in e), name
+        _same(g, e, name + " (graph after a layout rebuild)")

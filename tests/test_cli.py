@@ -62,8 +62,10 @@ def _write_simple(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("version", [1, 2])
+@pytest.mark.parametrize("version", [1, 2, 3])
 def test_end_to_end_simple(tmp_path, version):
+    """-V 3 is the CSR5 driver's signature (no test set): no per-iteration
+    RMSE line, the text-test RMSE of -p 1 only."""
     g, ds = _write_simple(tmp_path)
     k, T = int(g["k"]), int(g["outer"])
     # reference factors of the reference's own cdmf_ref for -r 1
@@ -73,9 +75,13 @@ def test_end_to_end_simple(tmp_path, version):
              "-V", str(version), str(tmp_path)])
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
+    assert "[info] Picked Version %d" % version in r.stdout
     lines = re.findall(r"\[-INFO-\] iteration num (\d+) .*RMSE=([0-9.]+)", r.stdout)
-    assert [int(a) for a, _ in lines] == list(range(1, T + 1))
-    np.testing.assert_allclose([float(b) for _, b in lines], g["rmse"], atol=2e-6)
+    if version == 3:
+        assert lines == [] and "[INFO] spmv time" in r.stdout
+    else:
+        assert [int(a) for a, _ in lines] == list(range(1, T + 1))
+        np.testing.assert_allclose([float(b) for _, b in lines], g["rmse"], atol=2e-6)
     assert re.search(r"\[INFO\] OCL Training time: [0-9.]+ s", r.stdout)
     m = re.search(r"\[INFO\] Test RMSE = ([0-9.]+)", r.stdout)
     assert m and abs(float(m.group(1)) - g["rmse"][-1]) < 2e-6

@@ -100,3 +100,40 @@ def test_batch_specs_are_valid_powerlaw_shapes():
     for bp in small:  # every drawn shape generates
         ds = D.batch_dataset(bp)
         assert ds.nnz + ds.nnz_test == bp.nnz
+
+
+@pytest.mark.parametrize("shape", ["a", "b", "c"])
+def test_stress_shapes_follow_the_reference_generators(shape):
+    """cdmf_gen_stress against the structure of data/{a,b,c}/{a,b,c}.py:
+    sizes, the dense row / column, two ratings on every other row (the fixed
+    column and one drawn from the stated range), train/100 test entries
+    apart from the training ratings, ratings in 1..5, and the same matrix
+    for the same seed."""
+    ds = D.gen_stress(shape, seed=0)
+    _check_compressed(ds)
+    m, n = {"a": (64000, 150), "b": (1024000, 1024000), "c": (1024000, 1024000)}[shape]
+    assert (ds.m, ds.n) == (m, n)
+    ntrain = n + 2 * (m - 1) if shape in "ab" else 2 * m
+    assert ds.nnz == ntrain and ds.nnz_test == ntrain // 100
+    rdeg = np.diff(ds.csr_ptr.astype(np.int64))
+    cdeg = np.diff(ds.csc_ptr.astype(np.int64))
+    dense_row = {"a": 0, "b": m - 1, "c": None}[shape]
+    dense_col = {"a": 0, "b": n - 1, "c": 0}[shape]
+    assert cdeg[dense_col] == m
+    sparse = np.ones(m, bool)
+    if dense_row is not None:
+        assert rdeg[dense_row] == n
+        sparse[dense_row] = False
+    assert np.all(rdeg[sparse] == 2)
+    rows = np.repeat(np.arange(m), rdeg)
+    drawn = ds.csr_idx[(rows != (dense_row if dense_row is not None else -1)) & (ds.csr_idx != dense_col)]
+    lo, hi = {"a": (1, n), "b": (0, n - 1), "c": (1, n)}[shape]
+    assert drawn.size == sparse.sum() and drawn.min() >= lo and drawn.max() < hi
+    train = set((rows.astype(np.int64) * n + ds.csr_idx).tolist())
+    test = set((ds.test_row.astype(np.int64) * n + ds.test_col).tolist())
+    assert len(test) == ds.nnz_test and not (train & test)
+    vals = np.concatenate([ds.csr_val, ds.test_val])
+    assert np.all(np.isin(vals, [1, 2, 3, 4, 5]))
+    again = D.gen_stress(shape, seed=0)
+    assert np.array_equal(again.csr_idx, ds.csr_idx) and np.array_equal(again.csr_val, ds.csr_val)
+    assert np.array_equal(again.test_col, ds.test_col)

@@ -391,7 +391,19 @@ def _holey(ds):
                                  (ds.test_row, ds.test_col, ds.test_val))
 
 
-@pytest.mark.parametrize("sigma", [8, 16])
+# Tiles of 64 x 32 elements: refused by ccd_set_plan.  Compiled for gfx950
+# the sigma-32 kernels spill past the 128 VGPRs of a 1024-thread block and
+# gave wrong results on every multi-tile case of these two tests (the
+# 2,000 x 1,000 case: max |dW| 1.02, max |dRMSE| 4.1e-3), while one-tile
+# cases passed.  The host tables and a restatement of the tile algorithm
+# are exact at sigma 32 under the address and undefined-behaviour
+# sanitizers (tests/test_tile_layout.py), so the fault is in the compiled
+# sigma-32 kernels; its cause is not shown, so sigma 32 stays off.
+SIGMA32_SKIP = pytest.mark.skip(reason="sigma 32 is refused: wrong results of the compiled sigma-32 tile kernels on "
+                                       "multi-tile layouts, cause not shown (host tables verified on the CPU)")
+
+
+@pytest.mark.parametrize("sigma", [8, 16, pytest.param(32, marks=SIGMA32_SKIP)])
 def test_tiled_variant(oracle_lib, sigma):
     from cdmf_amd import data as D
     cases = []
@@ -445,7 +457,7 @@ def _compare_tall(ds, got, ref, label):
     _close_bulk(got[4], ref[4], label + " residual CSC", quantile=0.9999)
 
 
-@pytest.mark.parametrize("sigma", [8, 16])
+@pytest.mark.parametrize("sigma", [8, 16, pytest.param(32, marks=SIGMA32_SKIP)])
 def test_tiled_3byte_indices(oracle_lib, sigma):
     ds = _tall((1 << 16) + 5000, 400, 30000, 12, seed=11)
     k, lam = 4, 0.05
@@ -583,6 +595,32 @@ def test_bf16_storage(oracle_lib, variant):
         variant, np.max(np.abs(np.array(rm) - rmb)), np.max(np.abs(np.array(rm) - rm64)), fw, fh))
     np.testing.assert_allclose(rm, rmb, rtol=0, atol=1e-3)
     assert fw <= 1e-2 and fh <= 1e-2
+
+
+@pytest.mark.parametrize("variant", [1, 3])
+def test_bf16_first_rank_matches_twin(oracle_lib, variant):
+    """One rank, one inner iteration in BF16 storage against the oracle's
+    BF16 twin, whose rounding points are pinned bit for bit by
+    tests/test_oracle.py.  Inputs are identical (R, the bf16-rounded initial
+    factors), so the sums differ only in order and the bf16 results agree to
+    the bit except where a value lies within FP32 rounding of a bf16 tie:
+    at least 99.5 % of H (the first sweep) and 95 % of W (the second, which
+    gathers H) bitwise equal, none more than one bf16 step (2^-7 relative)
+    apart in H.  A device path that rounded at other points -- not rounding
+    the initial factors, or keeping FP32 between sweeps -- moves about half
+    of the entries."""
+    from cdmf_amd import solver
+    ds = _small()
+    k, lam = 1, 0.05
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    W, H, _, _, _ = _run_prec(ds, W0, H0, k, lam, 1, 1, solver.PREC_BF16, variant=variant)
+    Wb, Hb, _, _, _ = oracle_lib.ccd(ds, W0, H0, k, lam, 1, 1, dtype="bf16")
+    eqh, eqw = np.mean(H == Hb), np.mean(W == Wb)
+    relh = np.max(np.abs(H.astype(np.float64) - Hb) / np.maximum(np.abs(Hb), 1e-30))
+    _say("[bf16 first rank v%d] bitwise equal: H %.5f W %.5f; max relative H difference %.3g" % (
+        variant, eqh, eqw, relh))
+    assert eqh >= 0.995 and eqw >= 0.95
+    assert relh <= 2.0 ** -7
 
 
 @pytest.mark.parametrize("variant", [1, 3])

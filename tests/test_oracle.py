@@ -2,8 +2,12 @@
 
 * Bit-for-bit against fixtures written by the compiled reference
   (tests/golden/make_golden.py runs the reference's cdmf_ref.cpp).
-* Functions with no reference counterpart (FP64 and BF16 twins, objective,
-  dual SpMV, FP64 sweep, FP64 residual) against mathematics: closed forms
+* The FP32 oracle and its BF16 twin against an independent, vectorised
+  numpy restatement with the same rounding points, bit for bit; the twin
+  built with identity rounding against the FP32 oracle, bit for bit; the
+  CPU-baseline slices (ccd_ranks) chained against the whole solve.
+* Functions with no reference counterpart (FP64 twin, objective, dual SpMV,
+  FP64 sweep, FP64 residual, train RMSE) against mathematics: closed forms
   recomputed independently in numpy/scipy, the transpose symmetry, a
   monotone objective, bf16 representability.
 """
@@ -183,19 +187,163 @@ def test_objective_non_increasing(oracle_lib):
     assert math.isclose(objs[-1], ref, rel_tol=1e-6)
 
 
-def test_bf16_twin(oracle_lib):
+def test_bf16_rounding_rule(oracle_lib):
     ds = _small()
     k = 4
     W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
     W, H, rb, _, _ = oracle_lib.ccd(ds, W0, H0, k, 0.05, 2, 2, dtype="bf16")
     for X in (W, H):
         assert np.all((X.view(np.uint32) & 0xFFFF) == 0)  # every stored value is a bf16
-    _, _, rf, _, _ = oracle_lib.ccd(ds, W0, H0, k, 0.05, 2, 2)
-    assert np.max(np.abs(rb - rf)) < 2e-2
     # rounding rule: nearest, ties to even
     assert oracle_lib.bf16(1.0 + 2 ** -8) == 1.0
     assert oracle_lib.bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
     assert oracle_lib.bf16(1.0 + 2 ** -8 + 2 ** -20) == 1.0 + 2 ** -7
+    x = np.random.default_rng(5).standard_normal(4096).astype(np.float32) * 10.0
+    assert np.array_equal(np.array([oracle_lib.bf16(float(v)) for v in x], np.float32), _bf16_np(x))
+
+
+# ------------------------------------------- independent numpy restatement
+
+def _bf16_np(x):
+    """Nearest bfloat16 (ties to even) by a route other than the oracle's bit
+    arithmetic: x = m 2^e with 0.5 <= |m| < 1, m rounded to 8 significant
+    bits by numpy's half-to-even rounding (normal numbers only)."""
+    m, e = np.frexp(np.asarray(x, dtype=np.float64))
+    return np.ldexp(np.round(m * 256.0) / 256.0, e).astype(np.float32)
+
+
+def _np_sweep(ptr, idx, val, x, lam32):
+    """cdmf_ref.cpp:7-17 over every row of a compressed layout, vectorised
+    across rows but serial in index order inside each row: g and h (seeded
+    with lambda * |row| as a float product) accumulate one element at a
+    time in float32; empty rows give 0."""
+    f32 = np.float32
+    deg = np.diff(ptr.astype(np.int64))
+    g = np.zeros(deg.size, f32)
+    h = (f32(lam32) * deg.astype(f32)).astype(f32)
+    for p in range(int(deg.max(initial=0))):
+        sel = np.flatnonzero(deg > p)
+        e = ptr[sel].astype(np.int64) + p
+        xi = x[idx[e]]
+        g[sel] = g[sel] + xi * val[e]
+        h[sel] = h[sel] + xi * xi
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(deg > 0, g / h, f32(0)).astype(f32)
+
+
+def _np_ccd(ds, W0, H0, k, lam, outer, inner, rnd=None):
+    """Independent numpy restatement of cdmf_ref.cpp:47-140 (H first, H
+    zeroed at entry), float32 throughout.  rnd: a rounding applied to the
+    initial factors and to every u / v value written (the BF16 twin).
+    Returns W, H and the two residual copies."""
+    f32 = np.float32
+    rnd = rnd or (lambda a: a)
+    W = rnd(np.array(W0, f32))
+    H = rnd(np.zeros_like(np.asarray(H0, f32)))
+    rv, kv = ds.csr_val.astype(f32).copy(), ds.csc_val.astype(f32).copy()
+    rrow = np.repeat(np.arange(ds.m), np.diff(ds.csr_ptr.astype(np.int64)))  # row of each CSR element
+    kcol = np.repeat(np.arange(ds.n), np.diff(ds.csc_ptr.astype(np.int64)))  # column of each CSC element
+    lam32 = f32(lam)
+    for oiter in range(1, outer + 1):
+        for t in range(k):
+            u, v = W[t].copy(), H[t].copy()
+            if oiter > 1:
+                kv[:] = kv + W[t][ds.csc_idx] * H[t][kcol]
+                rv[:] = rv + H[t][ds.csr_idx] * W[t][rrow]
+            for _ in range(inner):
+                v = rnd(_np_sweep(ds.csc_ptr, ds.csc_idx, kv, u, lam32))
+                u = rnd(_np_sweep(ds.csr_ptr, ds.csr_idx, rv, v, lam32))
+            W[t], H[t] = u, v
+            kv[:] = kv - u[ds.csc_idx] * v[kcol]
+            rv[:] = rv - v[ds.csr_idx] * u[rrow]
+    return W, H, rv, kv
+
+
+def _pin_cases():
+    from cdmf_amd import data as D
+    out = []
+    for name in ("simple", "tricky_simple"):
+        g = load_golden(name)
+        out.append((name, golden_dataset(g), int(g["k"]), float(g["lam"]), int(g["outer"]), int(g["inner"])))
+    out.append(("small", _small(), D.SMALL_SOLVE.k, D.SMALL_SOLVE.lam, 3, 3))
+    return out
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_numpy_restatement_equals_oracle_f32(oracle_lib, case):
+    """The FP32 oracle against a second, vectorised restatement: bitwise."""
+    name, ds, k, lam, outer, inner = _pin_cases()[case]
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    W, H, _, rv, kv = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner)
+    Wn, Hn, rvn, kvn = _np_ccd(ds, W0, H0, k, lam, outer, inner)
+    for a, b, what in ((W, Wn, "W"), (H, Hn, "H"), (rv, rvn, "CSR residual"), (kv, kvn, "CSC residual")):
+        assert np.array_equal(a, b), "%s %s" % (name, what)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_bf16_twin_equals_numpy_restatement(oracle_lib, case):
+    """The BF16 twin against the numpy restatement with its own bf16
+    rounding at the same points (initial factors, every u / v write):
+    bitwise.  Leaving out any one rounding point fails this."""
+    name, ds, k, lam, outer, inner = _pin_cases()[case]
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    W, H, rm, rv, kv = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner, dtype="bf16")
+    Wn, Hn, rvn, kvn = _np_ccd(ds, W0, H0, k, lam, outer, inner, rnd=_bf16_np)
+    for a, b, what in ((W, Wn, "W"), (H, Hn, "H"), (rv, rvn, "CSR residual"), (kv, kvn, "CSC residual")):
+        assert np.array_equal(a, b), "%s %s" % (name, what)
+    assert math.isclose(rm[-1], oracle_lib.rmse(Wn, Hn, ds), rel_tol=1e-15)
+    # the twin does round: it differs from the FP32 oracle
+    Wf, _, _, _, _ = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner)
+    assert not np.array_equal(W, Wf)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_bf16_twin_with_identity_rounding_is_the_oracle(oracle_lib, case):
+    """The BF16 twin compiled with its rounding replaced by the identity
+    (oracle/Makefile) reproduces the FP32 oracle bit for bit: apart from the
+    rounding it runs the same loop."""
+    name, ds, k, lam, outer, inner = _pin_cases()[case]
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    a = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner)
+    b = oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner, dtype="bf16", _lib=oracle_lib.lib_bf16_identity())
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y), name
+
+
+def test_ccd_ranks_chained_equal_ccd(oracle_lib):
+    """or_ccd_ranks (the CPU baseline) over ranks [0, 4) then [4, k) of
+    outer iteration 1, then of outer iteration 2, with the residual carried
+    in place, equals or_ccd with two outer iterations bit for bit."""
+    ds = _small()
+    k, lam, inner = 10, 0.05, 3
+    W0 = oracle_lib.initial_col(k, ds.m)
+    H0 = np.zeros((k, ds.n), np.float32)
+    st = oracle_lib.RankState(ds, W0, H0)
+    for oiter in (1, 2):
+        oracle_lib.ccd_ranks(st, 0, 4, lam, oiter, inner, threads=3)
+        oracle_lib.ccd_ranks(st, 4, k, lam, oiter, inner, threads=3)
+    W, H, _, rv, kv = oracle_lib.ccd(ds, W0, H0, k, lam, 2, inner)
+    for a, b, what in ((st.W, W, "W"), (st.H, H, "H"), (st.rv, rv, "CSR residual"), (st.kv, kv, "CSC residual")):
+        assert np.array_equal(a, b), what
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f64"])
+def test_train_rmse_per_outer_iteration(oracle_lib, dtype):
+    """train_out[i] is sqrt(mean(rhat^2)) of the residual left after outer
+    iteration i + 1, recomputed in numpy from a run stopped there; in FP32
+    it is also non-increasing here (the objective falls and lambda is
+    small)."""
+    ds = _small()
+    k, lam, outer, inner = 6, 0.05, 4, 2
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    tr = np.full(outer, np.nan)
+    oracle_lib.ccd(ds, W0, H0, k, lam, outer, inner, dtype=dtype, train_out=tr)
+    for it in range(1, outer + 1):
+        _, _, _, rv, kv = oracle_lib.ccd(ds, W0, H0, k, lam, it, inner, dtype=dtype)
+        ref = math.sqrt(np.mean(np.asarray(rv, np.float64) ** 2))
+        assert math.isclose(tr[it - 1], ref, rel_tol=1e-13), (it, tr[it - 1], ref)
+        assert math.isclose(ref, math.sqrt(np.mean(np.asarray(kv, np.float64) ** 2)), rel_tol=1e-13)
+    assert np.all(np.diff(tr) <= 1e-9), tr
 
 
 def test_dual_spmv_and_sweep_against_scipy(oracle_lib):
