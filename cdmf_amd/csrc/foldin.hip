@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -402,6 +403,51 @@ int32_t ccd_mlp_train(ccd_mlp* m, uint64_t rows, const float* X, const float* T,
 
 }  // extern "C"
 
+// ------------------------------------------------- allocation-free path ---
+// Scratch of ccd_fold_rows_noalloc: one call's staged batch -- row pointer,
+// long-row list, indices, values, the per-rating residual buffer of the long
+// rows, start factors and result factors -- carved from one file-scope
+// device array.  Every GPU the library runs on holds its own copy (about
+// 28 MB) until the process ends.  Calls on one GPU share it: calls that
+// overlap on one GPU race.  Nothing in it needs to start at zero (the sweep
+// kernel seeds the residual buffer from the values, and W from W0 or 0).
+namespace {
+constexpr uint64_t NA_MAX_ROWS = 4096;
+constexpr uint64_t NA_MAX_NNZ = 1ull << 21;
+constexpr size_t na_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_OFF_PTR = 0;
+constexpr size_t NA_OFF_LONG = NA_OFF_PTR + na_round(4 * (NA_MAX_ROWS + 1));
+constexpr size_t NA_OFF_IDX = NA_OFF_LONG + na_round(4 * NA_MAX_ROWS);
+constexpr size_t NA_OFF_VAL = NA_OFF_IDX + na_round(4 * NA_MAX_NNZ);
+constexpr size_t NA_OFF_RBUF = NA_OFF_VAL + na_round(4 * NA_MAX_NNZ);
+constexpr size_t NA_OFF_W0 = NA_OFF_RBUF + na_round(4 * NA_MAX_NNZ);
+constexpr size_t NA_OFF_W = NA_OFF_W0 + na_round(4 * NA_MAX_ROWS * ccd::FOLD_KMAX);
+constexpr size_t NA_BYTES = NA_OFF_W + na_round(4 * NA_MAX_ROWS * ccd::FOLD_KMAX);
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (the array's host-side name is no device
+// address).
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch;
+
+hipError_t scratch_on(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch.size()) g_scratch.resize((size_t)device + 1, nullptr);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+}  // namespace
+
 // -------------------------------------------------------------- fold-in ---
 struct ccd_fold {
     int device = 0;
@@ -416,12 +462,12 @@ struct ccd_fold {
     float *val = nullptr, *rbuf = nullptr, *W0 = nullptr, *W = nullptr, *F = nullptr;
     size_t cap_ptr = 0, cap_idx = 0, cap_val = 0, cap_rbuf = 0, cap_w0 = 0, cap_w = 0, cap_f = 0, cap_long = 0;
     hipEvent_t ea = nullptr, eb = nullptr;
+    unsigned char* scratch = nullptr;  // this device's scratch (ccd_fold_rows_noalloc)
 };
 
 namespace {
 
-int fold_upload(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
-                ccd::FoldArgs* a) {
+int fold_check(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val) {
     if (rows < 0 || rows > 0x7fffffff || !ptr || (rows && !idx && ptr[rows] > 0) || (rows && !val && ptr[rows] > 0))
         return obj_err(f->err, CCD_ERR_INVALID, "bad rows / null arrays");
     if (ptr[0] != 0) return obj_err(f->err, CCD_ERR_INVALID, "ptr[0] must be 0");
@@ -430,6 +476,13 @@ int fold_upload(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* 
     const uint64_t nnz = ptr[rows];
     for (uint64_t e = 0; e < nnz; ++e)
         if ((int64_t)idx[e] >= f->n) return obj_err(f->err, CCD_ERR_INVALID, "column index out of range");
+    return CCD_OK;
+}
+
+int fold_upload(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
+                ccd::FoldArgs* a) {
+    if (int rc = fold_check(f, rows, ptr, idx, val)) return rc;
+    const uint64_t nnz = ptr[rows];
     std::vector<uint32_t> lr;  // rows handled by a block each
     for (int64_t i = 0; i < rows; ++i)
         if (ptr[i + 1] - ptr[i] > ccd::FOLD_SHORT) lr.push_back((uint32_t)i);
@@ -472,7 +525,7 @@ int32_t ccd_fold_create(int32_t device, uint32_t k, int64_t n, const float* H, f
               dalloc(&f->H, kn) == hipSuccess && dalloc(&f->HT, kn) == hipSuccess &&
               ccd::copy_sync(f->stream, f->H, H, 4 * kn, hipMemcpyHostToDevice) == hipSuccess &&
               ccd::launch_transpose(f->H, f->HT, k, (uint32_t)n, f->stream) == hipSuccess &&
-              hipStreamSynchronize(f->stream) == hipSuccess;
+              hipStreamSynchronize(f->stream) == hipSuccess && scratch_on(device, &f->scratch) == hipSuccess;
     if (!ok) {
         ccd_fold_destroy(f);
         return CCD_ERR_HIP;
@@ -529,6 +582,50 @@ int32_t ccd_fold_rows(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint
     OBJ_TRY(f, hipEventElapsedTime(&ms, f->ea, f->eb));
     if (seconds) *seconds = ms * 1e-3;
     if (nw) OBJ_TRY(f, hipMemcpy(W, f->W, 4 * nw, hipMemcpyDeviceToHost));
+    return CCD_OK;
+}
+
+int32_t ccd_fold_rows_noalloc(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
+                              const float* W0, int32_t sweeps, float* W, double* seconds) {
+    if (!f || !W || sweeps < 0) return CCD_ERR_INVALID;
+    if (!f->scratch) return obj_err(f->err, CCD_ERR_STATE, "no scratch on this device");
+    if (int rc = fold_check(f, rows, ptr, idx, val)) return rc;
+    const uint64_t nnz = ptr[rows];
+    if ((uint64_t)rows > NA_MAX_ROWS || nnz > NA_MAX_NNZ)
+        return obj_err(f->err, CCD_ERR_INVALID,
+                       "ccd_fold_rows_noalloc: at most 4096 rows and 2097152 ratings per call (got " +
+                           std::to_string(rows) + " rows, " + std::to_string(nnz) + " ratings)");
+    // long rows (one block each), listed in a fixed-size host array
+    uint32_t lr[NA_MAX_ROWS];
+    uint32_t nlong = 0;
+    for (int64_t i = 0; i < rows; ++i)
+        if (ptr[i + 1] - ptr[i] > ccd::FOLD_SHORT) lr[nlong++] = (uint32_t)i;
+    unsigned char* S = f->scratch;
+    uint32_t* dptr = reinterpret_cast<uint32_t*>(S + NA_OFF_PTR);
+    uint32_t* dlong = reinterpret_cast<uint32_t*>(S + NA_OFF_LONG);
+    uint32_t* didx = reinterpret_cast<uint32_t*>(S + NA_OFF_IDX);
+    float* dval = reinterpret_cast<float*>(S + NA_OFF_VAL);
+    float* drbuf = reinterpret_cast<float*>(S + NA_OFF_RBUF);
+    float* dW0 = reinterpret_cast<float*>(S + NA_OFF_W0);
+    float* dW = reinterpret_cast<float*>(S + NA_OFF_W);
+    const size_t nw = (size_t)rows * f->k;
+    OBJ_TRY(f, hipSetDevice(f->device));
+    OBJ_TRY(f, ccd::copy_sync(f->stream, dptr, ptr, 4 * ((size_t)rows + 1), hipMemcpyHostToDevice));
+    if (nlong) OBJ_TRY(f, ccd::copy_sync(f->stream, dlong, lr, 4 * (size_t)nlong, hipMemcpyHostToDevice));
+    if (nnz) {
+        OBJ_TRY(f, ccd::copy_sync(f->stream, didx, idx, 4 * nnz, hipMemcpyHostToDevice));
+        OBJ_TRY(f, ccd::copy_sync(f->stream, dval, val, 4 * nnz, hipMemcpyHostToDevice));
+    }
+    if (W0 && nw) OBJ_TRY(f, ccd::copy_sync(f->stream, dW0, W0, 4 * nw, hipMemcpyHostToDevice));
+    const ccd::FoldArgs a{dptr, didx, dval, f->H, f->HT, (uint32_t)rows, f->k, (uint32_t)f->n, sweeps, f->lambda};
+    OBJ_TRY(f, hipEventRecord(f->ea, f->stream));
+    OBJ_TRY(f, ccd::launch_fold_sweeps(a, dlong, nlong, W0 ? dW0 : nullptr, dW, drbuf, f->stream));
+    OBJ_TRY(f, hipEventRecord(f->eb, f->stream));
+    OBJ_TRY(f, hipEventSynchronize(f->eb));
+    float ms = 0.0f;
+    OBJ_TRY(f, hipEventElapsedTime(&ms, f->ea, f->eb));
+    if (seconds) *seconds = ms * 1e-3;
+    if (nw) OBJ_TRY(f, hipMemcpy(W, dW, 4 * nw, hipMemcpyDeviceToHost));
     return CCD_OK;
 }
 

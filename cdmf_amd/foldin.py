@@ -50,6 +50,7 @@ def _lib():
             "ccd_fold_last_error": ([vp], C.c_char_p),
             "ccd_fold_features": ([vp, C.c_int64, u32p, u32p, f32p, f32p], C.c_int32),
             "ccd_fold_rows": ([vp, C.c_int64, u32p, u32p, f32p, f32p, C.c_int32, f32p, f64p], C.c_int32),
+            "ccd_fold_rows_noalloc": ([vp, C.c_int64, u32p, u32p, f32p, f32p, C.c_int32, f32p, f64p], C.c_int32),
             "ccd_fold_warm": ([vp, vp, C.c_int64, u32p, u32p, f32p, C.c_int32, C.c_int32, f32p, f64p], C.c_int32),
         }
         for name, (args, res) in sig.items():
@@ -63,7 +64,8 @@ def _lib():
 FOLDIN_SYMBOLS = ("ccd_mlp_create", "ccd_mlp_destroy", "ccd_mlp_last_error", "ccd_mlp_param_count",
                   "ccd_mlp_get_params", "ccd_mlp_set_params", "ccd_mlp_set_norm", "ccd_mlp_forward",
                   "ccd_mlp_loss_grad", "ccd_mlp_train", "ccd_fold_create", "ccd_fold_destroy",
-                  "ccd_fold_last_error", "ccd_fold_features", "ccd_fold_rows", "ccd_fold_warm")
+                  "ccd_fold_last_error", "ccd_fold_features", "ccd_fold_rows", "ccd_fold_rows_noalloc",
+                  "ccd_fold_warm")
 
 
 def _f32(a):
@@ -184,6 +186,19 @@ class FoldIn:
         s = C.c_double()
         self._check(self._lib.ccd_fold_rows(self._f, ptr.size - 1, N.ptr(ptr, C.c_uint32), N.ptr(idx, C.c_uint32),
                                             N.ptr(val, C.c_float), w0, sweeps, N.ptr(W, C.c_float), C.byref(s)))
+        return W, s.value
+
+    def rows_noalloc(self, ptr, idx, val, sweeps: int, W0=None):
+        """rows() through ccd_fold_rows_noalloc: no device allocation, at most
+        4096 rows and 2,097,152 ratings per call."""
+        ptr, idx, val = _u32(ptr), _u32(idx), _f32(val)
+        W = np.empty((ptr.size - 1, self.k), np.float32)
+        keep = _f32(W0) if W0 is not None else None
+        w0 = N.ptr(keep, C.c_float) if keep is not None else None
+        s = C.c_double()
+        self._check(self._lib.ccd_fold_rows_noalloc(self._f, ptr.size - 1, N.ptr(ptr, C.c_uint32),
+                                                    N.ptr(idx, C.c_uint32), N.ptr(val, C.c_float), w0, sweeps,
+                                                    N.ptr(W, C.c_float), C.byref(s)))
         return W, s.value
 
     def warm(self, mlp: MLP, ptr, idx, val, sweeps: int, bf16: bool = True):

@@ -65,6 +65,17 @@ int32_t ccd_fold_features(ccd_fold* f, int64_t rows, const uint32_t* ptr, const 
  * starting from W0 ([rows][k]; NULL = zeros).  *seconds: device time. */
 int32_t ccd_fold_rows(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
                       const float* W0, int32_t sweeps, float* W, double* seconds);
+/* ccd_fold_rows without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (serving loops folding in
+ * batches of new users, code that must not call hipMalloc).  Same kernels
+ * and result as ccd_fold_rows (bit for bit); the batch is staged in a
+ * library-owned static device scratch of about 28 MB per GPU, held until
+ * the process ends, so one call takes at most 4096 rows and 2,097,152
+ * ratings (larger batches fail with CCD_ERR_INVALID and a message).  Every
+ * call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (from several threads or fold objects) race. */
+int32_t ccd_fold_rows_noalloc(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
+                              const float* W0, int32_t sweeps, float* W, double* seconds);
 /* Warm start: features -> MLP (bfloat16 weights if bf16_weights) -> W0,
  * then `sweeps` sweeps; the MLP must map 2k + 2 inputs to k outputs.
  * *seconds: device time of the whole pipeline. */
