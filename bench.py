@@ -43,24 +43,26 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(ds, W0, k, lam, inner, budget_s=20.0):
+def cpu_baseline(ds, W0, k, lam, inner, budget_s=15.0):
     """OpenMP oracle (the restated cdmf_ref loops, schedule(dynamic,500)) on
-    the same matrix, one OpenMP thread per host core: whole ranks of a
-    steady-state outer iteration (add-back, 5 inner sweeps, subtract), as
-    many as fit in `budget_s`.  R's values and W, H are copied once, before
-    the clock starts; the ranks then carry the residual from one to the
-    next in place, as the solver does."""
+    the same matrix, one OpenMP thread per host core: whole ranks of
+    steady-state outer iterations (add-back, 5 inner sweeps, subtract; outer
+    iterations 2, 3, ...), as many as fit in `budget_s`.  R's values and W,
+    H are copied once, before the clock starts; the ranks then carry the
+    residual from one to the next in place, as the solver does."""
     import numpy as np
 
     import oracle
     threads = os.cpu_count() or 1
     H0 = np.zeros((k, ds.n), np.float32)
     st = oracle.RankState(ds, W0, H0)
-    ranks = 0
+    ranks, oiter = 0, 2
     t0 = time.perf_counter()
-    while ranks < k:
-        oracle.ccd_ranks(st, ranks, ranks + 1, lam, 2, inner, threads=threads)
+    while True:
+        oracle.ccd_ranks(st, ranks % k, ranks % k + 1, lam, oiter, inner, threads=threads)
         ranks += 1
+        if ranks % k == 0:
+            oiter += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
@@ -70,10 +72,10 @@ def cpu_baseline(ds, W0, k, lam, inner, budget_s=20.0):
         "cores": threads,
         "kind": "port",
         "cpu_model": cpu_model(),
-        "sample": "ranks 0..%d of one steady-state outer iteration (add-back, %d inner sweeps, subtract) on the "
+        "sample": "%d ranks (outer iterations 2.., %d ranks each: add-back, %d inner sweeps, subtract) on the "
                   "full %dx%d matrix, residual carried from rank to rank, %.1f s, %d OpenMP threads "
                   "(os.cpu_count()) on %s"
-                  % (ranks - 1, inner, ds.m, ds.n, dt, threads, cpu_model()),
+                  % (ranks, k, inner, ds.m, ds.n, dt, threads, cpu_model()),
     }
 
 
@@ -127,6 +129,9 @@ def main():
     ap.add_argument("--foldin", action="store_true",
                     help="the held-out-users fold-in study on the large shape (plain vs learned warm start)")
     ap.add_argument("--foldin-users", type=int, default=0, help="rows of the fold-in study's matrix (0: 480,189)")
+    ap.add_argument("--foldin-tol", type=float, default=1e-4,
+                    help="fold-in study: distance to the converged test RMSE that counts as reached")
+    ap.add_argument("--foldin-max-sweeps", type=int, default=60, help="fold-in study: longest sweep curve recorded")
     ap.add_argument("--probe", action="store_true",
                     help="run a 256 MiB stream-copy kernel first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -298,13 +303,18 @@ def bench_batch(args):
             s.set_factors(W0, H0)
 
     # sequential: one ccd_run per problem
+    per_problem = []
+
     def sequential():
         tot = 0.0
+        per_problem.clear()
         for s, bp in zip(solvers, specs):
             t0 = time.perf_counter()
             s.run(bp.solve.k, bp.solve.lam, bp.solve.outer, bp.solve.inner, variant=S.VARIANT_TILED,
                   rmse_every=False)
-            tot += time.perf_counter() - t0
+            dt = time.perf_counter() - t0
+            per_problem.append(dt)
+            tot += dt
         return tot
 
     reset()
@@ -321,7 +331,9 @@ def bench_batch(args):
     reset()
     trial_ms = b.tune()
     plan = b.plan()
-    log("autotuned plan %s (trial %.2f ms)" % (plan, trial_ms))
+    tune_times = b.tune_times()
+    log("autotuned plan %s (trial %.2f ms); whole runs with 0, 1, ... problems solved alone: %s ms" % (
+        plan, trial_ms, ["%.1f" % x for x in tune_times]))
     reset()
     b.run()  # warm-up
     reset()
@@ -361,6 +373,9 @@ def bench_batch(args):
         "speedup_vs_sequential": seq_s / graph_wall,
         "max_test_rmse_diff_vs_sequential": worst,
         "teams": teams,
+        "tune_whole_run_ms_by_solo": tune_times,
+        "problems": [dict(m=bp.m, n=bp.n, nnz=nnz, k=bp.solve.k, updates=bp.solve.outer * bp.solve.k * nnz,
+                          sequential_s=t) for bp, (_, _, nnz), t in zip(specs, inits, per_problem)],
     }
     print(json.dumps(out), flush=True)
 
@@ -377,7 +392,7 @@ def bench_foldin(args):
     from cdmf_amd import foldin as F
 
     out = {"metric": "fold-in sweeps / device time to the plain fold-in's test RMSE (warm start vs zero start)",
-           "n_gpus": 1, "dtype": "fp32 (MLP weights bf16)", "studies": {}}
+           "n_gpus": 1, "dtype": "fp32 (MLP inference: bf16 matrix cores, FP32 accumulation)", "studies": {}}
     for name, kind in (("independent ratings", 1), ("planted rank-8 ratings", 3)):
         spec = dict(D.NETFLIX, rating_kind=kind)
         if args.foldin_users:
@@ -388,7 +403,8 @@ def bench_foldin(args):
         ds = D.gen_powerlaw(**spec)
         log("%s: generated %dx%d nnz=%d in %.1fs" % (name, ds.m, ds.n, ds.nnz, time.perf_counter() - t))
         cfg = D.FOLDIN_SOLVE
-        r = F.experiment(ds, cfg.k, cfg.lam, cfg.outer, cfg.inner, sweeps=8, epochs=20, log=log)
+        r = F.experiment(ds, cfg.k, cfg.lam, cfg.outer, cfg.inner, sweeps=8, epochs=20, converge=150,
+                         rmse_tol=args.foldin_tol, max_sweeps=args.foldin_max_sweeps, log=log)
         rec = dict(users=ds.m, held_out=len(r.plain_rmse) and None, target_rmse=r.target,
                    plain_sweeps=r.plain_sweeps, warm_sweeps_to_target=r.warm_sweeps,
                    plain_time_to_target_s=r.plain_time[r.plain_sweeps],
@@ -396,7 +412,16 @@ def bench_foldin(args):
                    plain_rmse=r.plain_rmse, warm_rmse=r.warm_rmse, plain_time_s=r.plain_time, warm_time_s=r.warm_time,
                    objective_star=r.obj_star, plain_objective=r.plain_obj, warm_objective=r.warm_obj,
                    plain_sweeps_to_objective=r.plain_sweeps_obj, warm_sweeps_to_objective=r.warm_sweeps_obj,
-                   mlp_train_loss=r.train_loss, timings=r.timings)
+                   mlp_train_loss=r.train_loss, timings=r.timings,
+                   rmse_star=r.rmse_star, rmse_star_warm=r.rmse_star_warm, rmse_tol=r.rmse_tol,
+                   plain_sweeps_to_converged=r.plain_sweeps_conv, warm_sweeps_to_converged=r.warm_sweeps_conv,
+                   plain_time_to_converged_s=r.plain_time[r.plain_sweeps_conv] if r.plain_sweeps_conv >= 0 else None,
+                   warm_time_to_converged_s=r.warm_time[r.warm_sweeps_conv] if r.warm_sweeps_conv >= 0 else None,
+                   converged_by_tolerance={
+                       "%g" % tol: dict(plain_sweeps=ps, warm_sweeps=ws,
+                                        plain_time_s=r.plain_time[ps] if ps >= 0 else None,
+                                        warm_time_s=r.warm_time[ws] if ws >= 0 else None)
+                       for tol, (ps, ws) in r.conv_by_tol.items()})
         rec.pop("held_out")
         out["studies"][name] = rec
     print(json.dumps(out), flush=True)

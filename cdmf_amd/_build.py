@@ -36,6 +36,7 @@ HIP_SOURCES = [
     "kernels/batch_kernels.hip",
     "kernels/foldin_kernels.hip",
     "kernels/mlp_kernels.hip",
+    "kernels/mlp_mfma.hip",
     "ccd_abi.hip",
     "batch.hip",
     "foldin.hip",

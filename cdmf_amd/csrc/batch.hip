@@ -6,7 +6,9 @@
 #include "ccd_ctx.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -19,6 +21,9 @@ struct ccd_batch {
         ccd_params p;
     };
     std::vector<Item> items;
+    // problems of the persistent launch (indices into items); the others
+    // of a run are solved alone first (ccd_batch_plan.solo)
+    std::vector<size_t> active;
     ccd_batch_plan plan{16, 0, 1.0f, 0};
     int ncu = 0;
     int lds_cu = 0, lds_block = 0;  // LDS bytes of a CU and of a block
@@ -37,6 +42,7 @@ struct ccd_batch {
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     double last_seconds = 0.0;
     std::vector<uint32_t> team;  // blocks per problem of the last launch
+    std::vector<double> tune_solo_ms;  // the last tune's whole-run times, solo = 0, 1, ...
 };
 
 namespace {
@@ -100,7 +106,7 @@ ccd::BatchLayout batch_layout(const ccdi::DevLayout& L) {
 
 // Tiled layouts of every problem for sigma, pair buffers, FP32 state.
 int prepare(ccd_batch* b, int sigma) {
-    for (size_t i = 0; i < b->items.size(); ++i) {
+    for (size_t i : b->active) {
         ccd_ctx* c = b->items[i].ctx;
         c->prec = CCD_PREC_FP32;
         for (ccdi::DevLayout* L : {&c->csr, &c->csc}) {
@@ -133,17 +139,17 @@ struct Split {
 };
 
 Split split_grid(const ccd_batch* b, uint32_t blocks, float expo, int sigma) {
-    const size_t np = b->items.size();
+    const size_t np = b->active.size();
     const uint32_t C = blocks / 8;
     std::vector<uint32_t> nb(np, 1u), cap(np), xcd(np);
     std::vector<double> w(np);
     for (size_t i = 0; i < np; ++i) {
-        ccd_ctx* c = b->items[i].ctx;
+        ccd_ctx* c = b->items[b->active[i]].ctx;
         const uint64_t te = 64ull * (uint64_t)sigma;
         const uint64_t tiles = (c->nnz + te - 1) / te;
         cap[i] = (uint32_t)std::max<uint64_t>(1, (tiles + ccd::BATCH_BLOCK / 64 - 1) / (ccd::BATCH_BLOCK / 64));
         cap[i] = std::min(cap[i], C);
-        w[i] = std::pow(std::max(problem_work(b->items[i]), 1.0), (double)expo);
+        w[i] = std::pow(std::max(problem_work(b->items[b->active[i]]), 1.0), (double)expo);
     }
     auto grow = [&](uint32_t left, int only_xcd) {
         for (; left > 0; --left) {
@@ -183,11 +189,11 @@ Split split_grid(const ccd_batch* b, uint32_t blocks, float expo, int sigma) {
 // Upload the launch description (trial: the autotuner's short run).
 int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool trial, ccd::BatchArgs* args) {
     const std::vector<uint32_t>& nb = sp.nb;
-    const size_t np = b->items.size();
+    const size_t np = b->active.size();
     std::vector<ccd::BatchProblem> probs(np);
     std::vector<uint32_t> owner(blocks, 0xffffffffu);
     for (size_t i = 0; i < np; ++i) {
-        const ccd_batch::Item& it = b->items[i];
+        const ccd_batch::Item& it = b->items[b->active[i]];
         ccd_ctx* c = it.ctx;
         const bool hfirst = it.p.order == CCD_ORDER_H_FIRST;
         ccd::BatchProblem& P = probs[i];
@@ -253,7 +259,8 @@ int enqueue_run(ccd_batch* b, int sigma, const ccd::BatchArgs& a, bool trial) {
     BATCH_TRY(b, ccd::launch_fill(reinterpret_cast<float*>(b->d_barriers), 0.0f, a.nprob, s));
     BATCH_TRY(b, ccd::launch_fill(reinterpret_cast<float*>(b->d_status), 0.0f, 1, s));
     BATCH_TRY(b, ccd::launch_fill(reinterpret_cast<float*>(b->d_tickets), 0.0f, 8 * 16, s));
-    for (const ccd_batch::Item& it : b->items) {
+    for (size_t i : b->active) {
+        const ccd_batch::Item& it = b->items[i];
         ccd_ctx* c = it.ctx;
         ccdi::DevLayout& F = first_layout(c, it.p);
         ccdi::DevLayout& S = second_layout(c, it.p);
@@ -267,7 +274,8 @@ int enqueue_run(ccd_batch* b, int sigma, const ccd::BatchArgs& a, bool trial) {
     }
     BATCH_TRY(b, ccd::launch_batch(a, sigma, s));
     if (!trial) {
-        for (const ccd_batch::Item& it : b->items) {
+        for (size_t i : b->active) {
+            const ccd_batch::Item& it = b->items[i];
             ccd_ctx* c = it.ctx;
             ccdi::DevLayout& F = first_layout(c, it.p);
             ccdi::DevLayout& S = second_layout(c, it.p);
@@ -296,8 +304,8 @@ int grid_blocks(ccd_batch* b, int sigma, uint32_t want, uint32_t* out) {
     b->lds_floats = (uint32_t)(lds / sizeof(float));
     uint32_t most = (uint32_t)(bpc * b->ncu) & ~7u;
     uint32_t g = want ? std::min(want, most) & ~7u : most;
-    if (g < b->items.size())
-        return berr(b, CCD_ERR_UNSUPPORTED, "more problems (" + std::to_string(b->items.size()) +
+    if (g < b->active.size())
+        return berr(b, CCD_ERR_UNSUPPORTED, "more problems (" + std::to_string(b->active.size()) +
                                                 ") than co-resident blocks (" + std::to_string(g) + "): split the batch");
     *out = g;
     return CCD_OK;
@@ -311,6 +319,7 @@ int check_status(ccd_batch* b) {
     return CCD_OK;
 }
 
+// A short trial launch of the problems in b->active (no state change).
 int timed_trial(ccd_batch* b, int sigma, uint32_t blocks, float expo, double* ms) {
     if (int rc = prepare(b, sigma)) return rc;
     const Split sp = split_grid(b, blocks, expo, sigma);
@@ -398,6 +407,7 @@ int32_t ccd_batch_set_plan(ccd_batch* b, const ccd_batch_plan* pp) {
     if (!(pp->work_exponent > 0.0f && pp->work_exponent <= 2.0f))
         return berr(b, CCD_ERR_INVALID, "batch plan: work_exponent in (0, 2]");
     if (pp->blocks % 8u) return berr(b, CCD_ERR_INVALID, "batch plan: blocks must be 0 or a multiple of 8");
+    if (pp->solo > 64u) return berr(b, CCD_ERR_INVALID, "batch plan: solo must be at most 64");
     b->plan = *pp;
     drop_graph(b);
     return CCD_OK;
@@ -409,12 +419,98 @@ int32_t ccd_batch_get_plan(ccd_batch* b, ccd_batch_plan* out) {
     return CCD_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The `solo` problems of largest work (ties: lower index first) and the
+// rest, in index order.
+void choose_solo(ccd_batch* b, uint32_t solo, std::vector<size_t>& alone) {
+    const size_t np = b->items.size();
+    std::vector<size_t> order(np);
+    std::iota(order.begin(), order.end(), (size_t)0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return problem_work(b->items[x]) > problem_work(b->items[y]); });
+    const size_t ns = std::min<size_t>(solo, np);
+    alone.assign(order.begin(), order.begin() + ns);
+    std::sort(alone.begin(), alone.end());
+    std::vector<uint8_t> is_alone(np, 0);
+    for (size_t i : alone) is_alone[i] = 1;
+    b->active.clear();
+    for (size_t i = 0; i < np; ++i)
+        if (!is_alone[i]) b->active.push_back(i);
+}
+
+// One problem on the whole GPU: ccd_run of the tiled sweeps on the plain
+// tile layout at the batch's sigma, exactly what the persistent kernel
+// computes for it.  The context's own plan is put back afterwards.
+int run_alone(ccd_batch* b, size_t i, int sigma, bool graph) {
+    ccd_ctx* c = b->items[i].ctx;
+    const ccd_plan_params keep = c->plan;
+    c->plan.sigma = (uint32_t)sigma;
+    c->plan.split_rows = 0;
+    ccd_params p = b->items[i].p;
+    p.variant = CCD_VARIANT_TILED;
+    p.precision = CCD_PREC_FP32;
+    p.rmse_every = 0;
+    p.profile = 0;
+    p.use_graph = graph ? 1 : 0;
+    const int rc = ccd_run(c, &p, nullptr, nullptr);
+    c->plan = keep;
+    return rc ? ctx_err(b, i, rc) : CCD_OK;
+}
+
+// Device copies of each problem's factors, to put back the start of a run
+// (tuner runs); the residual goes back to R as in ccd_set_factors.
+struct Snapshot {
+    std::vector<float*> W, H;
+};
+
+int take_snapshot(ccd_batch* b, Snapshot& sn) {
+    for (const ccd_batch::Item& it : b->items) {
+        ccd_ctx* c = it.ctx;
+        float *w = nullptr, *h = nullptr;
+        BATCH_TRY(b, ccdi::dalloc(&w, (size_t)c->k * c->m));
+        sn.W.push_back(w);
+        BATCH_TRY(b, ccdi::dalloc(&h, (size_t)c->k * c->n));
+        sn.H.push_back(h);
+        BATCH_TRY(b, ccd::copy_sync(c->stream, w, c->W, 4 * (size_t)c->k * c->m, hipMemcpyDeviceToDevice));
+        BATCH_TRY(b, ccd::copy_sync(c->stream, h, c->H, 4 * (size_t)c->k * c->n, hipMemcpyDeviceToDevice));
+    }
+    return CCD_OK;
+}
+
+int restore_snapshot(ccd_batch* b, const Snapshot& sn) {
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        ccd_ctx* c = b->items[i].ctx;
+        BATCH_TRY(b, ccd::copy_sync(c->stream, c->W, sn.W[i], 4 * (size_t)c->k * c->m, hipMemcpyDeviceToDevice));
+        BATCH_TRY(b, ccd::copy_sync(c->stream, c->H, sn.H[i], 4 * (size_t)c->k * c->n, hipMemcpyDeviceToDevice));
+        BATCH_TRY(b, ccd::copy_sync(c->stream, c->csr.val, c->csr.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
+        BATCH_TRY(b, ccd::copy_sync(c->stream, c->csc.val, c->csc.val0, 4 * c->nnz, hipMemcpyDeviceToDevice));
+        c->continued = false;
+    }
+    return CCD_OK;
+}
+
+void free_snapshot(Snapshot& sn) {
+    for (float* p : sn.W) ccdi::dfree(p);
+    for (float* p : sn.H) ccdi::dfree(p);
+    sn.W.clear();
+    sn.H.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
 int32_t ccd_batch_tune(ccd_batch* b, double* best_ms) {
     if (!b) return CCD_ERR_INVALID;
     if (b->items.empty()) return berr(b, CCD_ERR_STATE, "empty batch");
     BATCH_TRY(b, hipSetDevice(b->device));
     ccd_batch_plan best = b->plan;
     double bms = INFINITY;
+    b->active.resize(b->items.size());
+    std::iota(b->active.begin(), b->active.end(), (size_t)0);
     for (uint32_t sigma : {16u, 8u}) {
         uint32_t blocks = 0;
         if (int rc = grid_blocks(b, (int)sigma, 0, &blocks)) return rc;
@@ -426,6 +522,42 @@ int32_t ccd_batch_tune(ccd_batch* b, double* best_ms) {
                 best = ccd_batch_plan{sigma, 0u, expo, 0u};
             }
         }
+    }
+    // Solving the largest problems alone, on the whole GPU, before the
+    // persistent launch takes the rest: each team of the launch is confined
+    // to one XCD, so a problem with more than an eighth of the work bounds
+    // the launch.  Whole runs of each candidate, from the problems' start
+    // state (put back after every run), twice each (the first warms up);
+    // only from a fresh start (no problem continues a run).
+    bool fresh = b->items.size() > 1;
+    for (const ccd_batch::Item& it : b->items) fresh = fresh && !it.ctx->continued;
+    b->tune_solo_ms.clear();
+    if (fresh) {
+        Snapshot sn;
+        int rc = take_snapshot(b, sn);
+        double best_s = INFINITY;
+        uint32_t best_solo = 0;
+        const uint32_t most = (uint32_t)std::min<size_t>(3, b->items.size() - 1);
+        for (uint32_t solo = 0; !rc && solo <= most; ++solo) {
+            b->plan = best;
+            b->plan.solo = solo;
+            double secs = 0.0;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                drop_graph(b);
+                rc = ccd_batch_run(b, 0);
+                if (!rc) secs = b->last_seconds;
+                if (!rc) rc = restore_snapshot(b, sn);
+            }
+            if (!rc) b->tune_solo_ms.push_back(1e3 * secs);
+            if (!rc && secs < best_s) {
+                best_s = secs;
+                best_solo = solo;
+            }
+        }
+        free_snapshot(sn);
+        if (rc) return rc;
+        best.solo = best_solo;
+        bms = 1e3 * best_s;
     }
     b->plan = best;
     drop_graph(b);
@@ -441,22 +573,33 @@ int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph) {
     for (size_t i = 0; i < b->items.size(); ++i)
         if (b->items[i].p.zero_first && b->items[i].ctx->continued)
             return berr(b, CCD_ERR_INVALID, "problem " + std::to_string(i) + ": zero_first on a continued context");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<size_t> alone;
+    choose_solo(b, b->plan.solo, alone);
+    for (size_t i : alone)
+        if (int rc = run_alone(b, i, sigma, use_graph != 0)) return rc;
+    const double alone_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    b->team.assign(b->items.size(), 0u);
+    b->last_seconds = alone_s;
+    if (b->active.empty()) return CCD_OK;
     if (int rc = prepare(b, sigma)) return rc;
     uint32_t blocks = 0;
     if (int rc = grid_blocks(b, sigma, b->plan.blocks, &blocks)) return rc;
     const Split sp = split_grid(b, blocks, b->plan.work_exponent, sigma);
     ccd::BatchArgs a{};
     if (int rc = describe(b, sigma, blocks, sp, false, &a)) return rc;
-    b->team = sp.nb;
+    for (size_t q = 0; q < b->active.size(); ++q) b->team[b->active[q]] = sp.nb[q];
 
-    // the captured sequence depends on the plan, the grid, each problem's
-    // continued / zero_first state and the buffers it was captured with:
-    // a context's generation moves whenever it reallocates any of them
-    // (upload, a new k, a rebuilt tile layout), and the addresses the
-    // capture bakes in are keyed as well
-    std::vector<uint64_t> key{(uint64_t)sigma, blocks};
-    for (const ccd_batch::Item& it : b->items) {
+    // the captured sequence depends on the plan, the grid, the problems of
+    // the launch, each one's continued / zero_first state and the buffers
+    // it was captured with: a context's generation moves whenever it
+    // reallocates any of them (upload, a new k, a rebuilt tile layout), and
+    // the addresses the capture bakes in are keyed as well
+    std::vector<uint64_t> key{(uint64_t)sigma, blocks, (uint64_t)b->active.size()};
+    for (size_t i : b->active) {
+        const ccd_batch::Item& it = b->items[i];
         const ccd_ctx* c = it.ctx;
+        key.push_back(i);
         key.push_back((it.ctx->continued ? 1u : 0u) | (it.p.zero_first ? 2u : 0u));
         key.push_back(c->gen);
         for (const void* q : {(const void*)c->W, (const void*)c->H, (const void*)c->u, (const void*)c->v,
@@ -490,9 +633,9 @@ int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph) {
     if (int rc = check_status(b)) return rc;
     float ms = 0.0f;
     BATCH_TRY(b, hipEventElapsedTime(&ms, b->ev_a, b->ev_b));
-    b->last_seconds = ms * 1e-3;
-    for (const ccd_batch::Item& it : b->items)
-        if (it.p.maxiter > 0) it.ctx->continued = true;
+    b->last_seconds = alone_s + ms * 1e-3;
+    for (size_t i : b->active)
+        if (b->items[i].p.maxiter > 0) b->items[i].ctx->continued = true;
     return CCD_OK;
 }
 
@@ -500,6 +643,13 @@ int32_t ccd_batch_time(ccd_batch* b, double* seconds) {
     if (!b || !seconds) return CCD_ERR_INVALID;
     *seconds = b->last_seconds;
     return CCD_OK;
+}
+
+int32_t ccd_batch_tune_times(ccd_batch* b, double* ms, int32_t max_count) {
+    if (!b || !ms) return CCD_ERR_INVALID;
+    const int32_t n = std::min<int32_t>(max_count, (int32_t)b->tune_solo_ms.size());
+    for (int32_t i = 0; i < n; ++i) ms[i] = b->tune_solo_ms[i];
+    return (int32_t)b->tune_solo_ms.size();
 }
 
 int32_t ccd_batch_teams(ccd_batch* b, uint32_t* blocks, int32_t max_count) {

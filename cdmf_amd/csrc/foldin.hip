@@ -72,6 +72,11 @@ struct ccd_mlp {
     float* P = nullptr;  // FP32 master parameters
     uint16_t* P16 = nullptr;
     bool p16_valid = false;
+    // packed bfloat16 weights of the matrix-core inference path (mlp_mfma.hip)
+    uint16_t* pk = nullptr;
+    float* pkb = nullptr;
+    uint32_t kin = 0, nout = 0;
+    bool pk_valid = false;
     float *G = nullptr, *mom = nullptr, *vel = nullptr;
     uint64_t step = 0;
     // affine maps: inputs x_n = (x - in_sub) in_mul; outputs y = (y_n - out_sub) out_mul;
@@ -181,8 +186,47 @@ int mlp_sync_bf16(ccd_mlp* m, hipStream_t s) {
     return CCD_OK;
 }
 
-// Y (device, raw) = MLP(X (device, raw)) for any row count, chunked by the workspace.
-int mlp_forward_dev(ccd_mlp* m, uint64_t rows, const float* X, float* Y, bool b16, hipStream_t s) {
+// Packed weights of the matrix-core path (allocated on first use).
+int mlp_sync_mfma(ccd_mlp* m, hipStream_t s) {
+    if (m->hid != ccd::MLP_MFMA_HIDDEN || ((m->in + 31) & ~31u) > ccd::MLP_MFMA_MAX_KIN ||
+        m->out > ccd::MLP_MFMA_HIDDEN)
+        return obj_err(m->err, CCD_ERR_UNSUPPORTED, "matrix-core inference needs 256 hidden units, in <= 288, out <= 256");
+    if (m->pk_valid) return CCD_OK;
+    if (!m->pk) {
+        m->kin = (m->in + 31) & ~31u;
+        m->nout = (m->out + 15) & ~15u;
+        const size_t H = ccd::MLP_MFMA_HIDDEN;
+        OBJ_TRY(m, dalloc(&m->pk, H * m->kin + H * H + (size_t)m->nout * H));
+        OBJ_TRY(m, dalloc(&m->pkb, 2 * H + m->nout));
+    }
+    const size_t H = ccd::MLP_MFMA_HIDDEN;
+    uint16_t* W1 = m->pk;
+    uint16_t* W2 = W1 + H * m->kin;
+    uint16_t* W3 = W2 + H * H;
+    OBJ_TRY(m, ccd::launch_mlp_pack(m->P, m->in, m->out, m->kin, m->nout, W1, W2, W3, m->pkb, m->pkb + H,
+                                    m->pkb + 2 * H, s));
+    m->pk_valid = true;
+    return CCD_OK;
+}
+
+// Y (device, raw) = MLP(X (device, raw)) for any row count.  mode 0: FP32;
+// 1: bfloat16 weights, FP32 arithmetic (chunked GEMMs through the
+// workspace); 2: the matrix-core path, one launch (bfloat16 weights and
+// activations, FP32 accumulation).
+int mlp_forward_dev(ccd_mlp* m, uint64_t rows, const float* X, float* Y, int mode, hipStream_t s) {
+    if (mode == 2) {
+        if (int rc = mlp_sync_mfma(m, s)) return rc;
+        if (rows > 0xffffffffull) return obj_err(m->err, CCD_ERR_INVALID, "too many rows");
+        const size_t H = ccd::MLP_MFMA_HIDDEN;
+        ccd::MlpInferArgs g{};
+        g.X = X, g.Y = Y, g.rows = (uint32_t)rows, g.in = m->in, g.out = m->out, g.kin = m->kin, g.nout = m->nout;
+        g.W1 = m->pk, g.W2 = m->pk + H * m->kin, g.W3 = m->pk + H * m->kin + H * H;
+        g.b1 = m->pkb, g.b2 = m->pkb + H, g.b3 = m->pkb + 2 * H;
+        if (m->has_norm) g.in_sub = m->in_sub, g.in_mul = m->in_mul, g.out_sub = m->out_sub, g.out_mul = m->out_mul;
+        OBJ_TRY(m, ccd::launch_mlp_infer(g, s));
+        return CCD_OK;
+    }
+    const bool b16 = mode == 1;
     const uint32_t chunk = 16384;
     if (int rc = mlp_workspace(m, chunk)) return rc;
     if (b16)
@@ -258,7 +302,8 @@ int32_t ccd_mlp_destroy(ccd_mlp* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* bufs[] = {m->P, m->P16, m->G, m->mom, m->vel, m->in_sub, m->in_mul, m->out_sub, m->out_mul, m->tgt_sub,
                     m->tgt_mul, m->gpart, m->Xn, m->Z1,
-                    m->A1, m->Z2, m->A2, m->Yn, m->Tn, m->dY, m->dZ2, m->dZ1, m->part, m->loss, m->X, m->T, m->perm};
+                    m->A1, m->Z2, m->A2, m->Yn, m->Tn, m->dY, m->dZ2, m->dZ1, m->part, m->loss, m->X, m->T, m->perm,
+                    m->pk, m->pkb};
     for (void* b : bufs) dfree(b);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -283,6 +328,7 @@ int32_t ccd_mlp_set_params(ccd_mlp* m, const float* P) {
     OBJ_TRY(m, hipStreamSynchronize(m->stream));
     OBJ_TRY(m, ccd::copy_sync(m->stream, m->P, P, sizeof(float) * m->np, hipMemcpyHostToDevice));
     m->p16_valid = false;
+    m->pk_valid = false;
     return CCD_OK;
 }
 
@@ -329,7 +375,8 @@ int32_t ccd_mlp_forward(ccd_mlp* m, uint64_t rows, const float* X, float* Y, int
     }
     int rc = CCD_OK;
     if (ccd::copy_sync(m->stream, dX, X, 4 * rows * m->in, hipMemcpyHostToDevice) != hipSuccess) rc = CCD_ERR_HIP;
-    if (!rc) rc = mlp_forward_dev(m, rows, dX, dYo, bf16_weights != 0, m->stream);
+    if (bf16_weights < 0 || bf16_weights > 2) rc = obj_err(m->err, CCD_ERR_INVALID, "bf16_weights must be 0, 1 or 2");
+    if (!rc) rc = mlp_forward_dev(m, rows, dX, dYo, bf16_weights, m->stream);
     if (!rc && hipStreamSynchronize(m->stream) != hipSuccess) rc = CCD_ERR_HIP;
     if (!rc && hipMemcpy(Y, dYo, 4 * rows * m->out, hipMemcpyDeviceToHost) != hipSuccess) rc = CCD_ERR_HIP;
     dfree(dX);
@@ -397,6 +444,7 @@ int32_t ccd_mlp_train(ccd_mlp* m, uint64_t rows, const float* X, const float* T,
         last /= (double)nb;
     }
     m->p16_valid = false;
+    m->pk_valid = false;
     if (epoch_loss) *epoch_loss = last;
     return CCD_OK;
 }
@@ -645,7 +693,8 @@ int32_t ccd_fold_warm(ccd_fold* f, ccd_mlp* m, int64_t rows, const uint32_t* ptr
     OBJ_TRY(f, hipStreamSynchronize(m->stream));
     OBJ_TRY(f, hipEventRecord(f->ea, f->stream));
     OBJ_TRY(f, ccd::launch_fold_features(a, f->long_rows, f->nlong, f->F, f->stream));
-    if (int rc = mlp_forward_dev(m, (uint64_t)rows, f->F, f->W0, bf16_weights != 0, f->stream))
+    if (bf16_weights < 0 || bf16_weights > 2) return obj_err(f->err, CCD_ERR_INVALID, "bf16_weights must be 0, 1 or 2");
+    if (int rc = mlp_forward_dev(m, (uint64_t)rows, f->F, f->W0, bf16_weights, f->stream))
         return obj_err(f->err, rc, "MLP: " + m->err);
     OBJ_TRY(f, ccd::launch_fold_sweeps(a, f->long_rows, f->nlong, f->W0, f->W, f->rbuf, f->stream));
     OBJ_TRY(f, hipEventRecord(f->eb, f->stream));

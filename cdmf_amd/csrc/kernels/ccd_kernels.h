@@ -509,6 +509,28 @@ struct GemmArgs {
     uint64_t part_cap; // bias) with few output tiles splits K over slices, summed in slice order
 };
 hipError_t launch_gemm(const GemmArgs& g, const void* B, bool bf16_b, int epi, hipStream_t s);
+
+// Whole-network inference on the bfloat16 matrix cores (mlp_mfma.hip):
+// Y [rows][out] = out_map(W3 gelu(W2 gelu(W1 in_map(X) + b1) + b2) + b3),
+// hidden width MLP_MFMA_HIDDEN, inputs and activations rounded to bfloat16,
+// FP32 accumulation.  Weights packed by launch_mlp_pack: W1 [hidden][kin],
+// W2 [hidden][hidden], W3 [nout][hidden] bfloat16 with kin = in rounded up
+// to 32 and nout = out rounded up to 16 (zero padding), biases FP32 padded
+// likewise.  in_map: x -> (x - in_sub) in_mul, out_map: y -> (y - out_sub)
+// out_mul, identity where the pointers are null.
+constexpr uint32_t MLP_MFMA_HIDDEN = 256;
+constexpr uint32_t MLP_MFMA_MAX_KIN = 288;
+struct MlpInferArgs {
+    const float* X;
+    float* Y;
+    uint32_t rows, in, out, kin, nout;
+    const uint16_t *W1, *W2, *W3;
+    const float *b1, *b2, *b3;
+    const float *in_sub, *in_mul, *out_sub, *out_mul;
+};
+hipError_t launch_mlp_infer(const MlpInferArgs& g, hipStream_t s);
+hipError_t launch_mlp_pack(const float* P, uint32_t in, uint32_t out, uint32_t kin, uint32_t nout, uint16_t* W1,
+                           uint16_t* W2, uint16_t* W3, float* b1, float* b2, float* b3, hipStream_t s);
 hipError_t launch_colsum(const float* dZ, uint32_t M, uint32_t N, float* out, hipStream_t s);
 // dY = 2 (Y - T) / count; *out = mean (Y - T)^2 (FP64, fixed order); part: MSE_PARTS doubles
 constexpr uint32_t MSE_PARTS = 1024;

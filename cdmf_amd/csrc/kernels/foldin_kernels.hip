@@ -4,10 +4,10 @@
 //   a = r^ + w_t h_t,  w_t = sum a h_t / (lambda |row| + sum h_t^2),  r^ = a - w_t h_t
 // (the u-side update of src/cdmf_ref.cpp:7-17 on one row).
 //
-// One launch of 1024-thread blocks serves both row classes: blocks below
-// nlong take one long row each (more than FOLD_SHORT ratings, listed by the
-// host) with all 16 waves; the other blocks take 16 short rows, one per
-// wave.  Residual, indices and gathered h values stay in registers for the
+// The sweeps take one launch of 1024-thread blocks for both row classes:
+// blocks below nlong take one long row each (more than FOLD_SHORT ratings,
+// listed by the host) with all 16 waves; the other blocks take 16 short
+// rows, one per wave.  The features take a launch per class.  Residual, indices and gathered h values stay in registers for the
 // whole solve (rows above 16384 ratings keep the rest in a global buffer).
 // Per-row sums are reduced in a fixed order: deterministic.
 #include "ccd_kernels.h"
@@ -21,7 +21,7 @@ constexpr int LBLOCK = 1024;
 constexpr int LWAVES = LBLOCK / WAVE;
 constexpr int FOLD_E = 16;   // elements per lane of a short row (1024 per row)
 constexpr int LONG_E = 16;   // elements per thread of a long row (16384 per row)
-constexpr int FEAT_U = 16;   // elements in flight per feature step
+constexpr int FEAT_U = 32;   // H-row loads in flight per feature step
 static_assert(WAVE * FOLD_E == (int)FOLD_SHORT, "short-row bound of the host's long-row list");
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -30,34 +30,46 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Feature sums of elements q = s + first, s + first + step, ... < e for
-// rank t: S = sum r, m = sum r h, v = sum r h^2 (FP64, element order within
-// the slice, FEAT_U independent loads in flight).
-__device__ __forceinline__ void feature_sums(const FoldArgs& a, uint32_t s, uint32_t e, uint32_t first, uint32_t step,
+// Feature sums of a row's elements s .. e - 1 for rank t, over the chunks
+// of 64 elements c0, c0 + cstep, ...: S = sum r, m = sum r h, v = sum r h^2
+// (FP64, in element order within each chunk, chunks in order).  A chunk's
+// ratings and indices come in with one coalesced, non-temporal load each
+// (so that they do not push the item rows of HT, reread for every rating
+// of the item, out of L2); each element's index and rating are then read
+// from its lane (v_readlane) and the H rows of 32 elements are loaded
+// together.  Lanes past k load rank 0's entry and drop it, so there is no
+// branch around the loads.
+__device__ __forceinline__ void feature_sums(const FoldArgs& a, uint32_t s, uint32_t e, uint32_t c0, uint32_t cstep,
                                              uint32_t t, bool tok, double& S, double& m, double& v) {
     const uint32_t k = a.k;
-    uint32_t q = s + first;
-    for (; q + (FEAT_U - 1) * step < e; q += FEAT_U * step) {
-        float h[FEAT_U], r[FEAT_U];
+    const uint32_t tt = tok ? t : 0u;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (uint32_t base = s + c0 * WAVE; base < e; base += cstep * WAVE) {
+        const uint32_t p = base + (uint32_t)lane;
+        const bool okp = p < e;
+        const uint32_t cv = okp ? __builtin_nontemporal_load(a.idx + p) : 0u;
+        const float rv = okp ? __builtin_nontemporal_load(a.val + p) : 0.0f;
+        const uint32_t n = e - base < (uint32_t)WAVE ? e - base : (uint32_t)WAVE;
 #pragma unroll
-        for (int u = 0; u < FEAT_U; ++u) {
-            const uint32_t qq = q + u * step;
-            r[u] = a.val[qq];
-            h[u] = tok ? a.HT[(size_t)a.idx[qq] * k + t] : 0.0f;
-        }
+        for (int half = 0; half < WAVE / FEAT_U; ++half) {
+            if ((uint32_t)(FEAT_U * half) >= n) break;
+            float h[FEAT_U];
 #pragma unroll
-        for (int u = 0; u < FEAT_U; ++u) {
-            S += r[u];
-            m += (double)r[u] * h[u];
-            v += (double)r[u] * h[u] * h[u];
+            for (int u = 0; u < FEAT_U; ++u) {
+                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cv, FEAT_U * half + u);
+                h[u] = a.HT[(size_t)c * k + tt];
+            }
+#pragma unroll
+            for (int u = 0; u < FEAT_U; ++u) {
+                if ((uint32_t)(FEAT_U * half + u) < n) {
+                    const float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rv), FEAT_U * half + u));
+                    const float hu = tok ? h[u] : 0.0f;
+                    S += r;
+                    m += (double)r * hu;
+                    v += (double)r * hu * hu;
+                }
+            }
         }
-    }
-    for (; q < e; q += step) {
-        const float r = a.val[q];
-        const float h = tok ? a.HT[(size_t)a.idx[q] * k + t] : 0.0f;
-        S += r;
-        m += (double)r * h;
-        v += (double)r * h * h;
     }
 }
 
@@ -76,48 +88,53 @@ __device__ __forceinline__ void write_features(float* f, uint32_t k, uint32_t t,
     f[k + t] = (float)var;
 }
 
-// F [rows][2k + 2]: lanes take ranks t0 + lane; a short row is one wave's,
-// a long row's elements are dealt to the 16 waves and the per-wave sums
-// added in wave order.
-__global__ __launch_bounds__(LBLOCK) void fold_features_kernel(FoldArgs a, const uint32_t* __restrict__ long_rows,
-                                                               uint32_t nlong, float* __restrict__ F) {
+// F [rows][2k + 2] of the long rows (more than FOLD_SHORT ratings), one
+// 1024-thread block each: lanes take ranks t0 + lane, the row's elements are
+// dealt to the 16 waves and the per-wave sums added in wave order.
+__global__ __launch_bounds__(LBLOCK) void fold_features_long_kernel(FoldArgs a, const uint32_t* __restrict__ long_rows,
+                                                                    float* __restrict__ F) {
     __shared__ double part[3][LWAVES][WAVE];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     const uint32_t k = a.k;
-    if (blockIdx.x < nlong) {
-        const uint32_t row = long_rows[blockIdx.x];
-        const uint32_t s = a.ptr[row], e = a.ptr[row + 1], d = e - s;
-        float* f = F + (size_t)row * (2 * k + 2);
-        double Sr = 0.0;
-        for (uint32_t t0 = 0; t0 < k; t0 += WAVE) {
-            const uint32_t t = t0 + lane;
-            const bool tok = t < k;
-            double S = 0.0, m = 0.0, v = 0.0;
-            feature_sums(a, s, e, (uint32_t)wv, LWAVES, t, tok, S, m, v);
-            part[0][wv][lane] = S;
-            part[1][wv][lane] = m;
-            part[2][wv][lane] = v;
-            __syncthreads();
-            if (wv == 0) {
-                double St = 0.0, mt = 0.0, vt = 0.0;
-                for (int w = 0; w < LWAVES; ++w) {
-                    St += part[0][w][lane];
-                    mt += part[1][w][lane];
-                    vt += part[2][w][lane];
-                }
-                write_features(f, k, t, tok, d, St, mt, vt);
-                Sr = St;
+    const uint32_t row = long_rows[blockIdx.x];
+    const uint32_t s = a.ptr[row], e = a.ptr[row + 1], d = e - s;
+    float* f = F + (size_t)row * (2 * k + 2);
+    double Sr = 0.0;
+    for (uint32_t t0 = 0; t0 < k; t0 += WAVE) {
+        const uint32_t t = t0 + lane;
+        const bool tok = t < k;
+        double S = 0.0, m = 0.0, v = 0.0;
+        feature_sums(a, s, e, (uint32_t)wv, LWAVES, t, tok, S, m, v);  // wave wv: chunks wv, wv + 16, ...
+        part[0][wv][lane] = S;
+        part[1][wv][lane] = m;
+        part[2][wv][lane] = v;
+        __syncthreads();
+        if (wv == 0) {
+            double St = 0.0, mt = 0.0, vt = 0.0;
+            for (int w = 0; w < LWAVES; ++w) {
+                St += part[0][w][lane];
+                mt += part[1][w][lane];
+                vt += part[2][w][lane];
             }
-            __syncthreads();
+            write_features(f, k, t, tok, d, St, mt, vt);
+            Sr = St;
         }
-        if (threadIdx.x == 0) {
-            f[2 * k] = (float)log1p((double)d);
-            f[2 * k + 1] = d ? (float)(Sr / d) : 0.0f;
-        }
-        return;
+        __syncthreads();
     }
-    const uint32_t row = (blockIdx.x - nlong) * LWAVES + wv;
-    if (row >= a.rows) return;
+    if (threadIdx.x == 0) {
+        f[2 * k] = (float)log1p((double)d);
+        f[2 * k + 1] = d ? (float)(Sr / d) : 0.0f;
+    }
+}
+
+// F of the other rows, one single-wave block per row (so a wave that
+// finishes a short row frees its slot at once, whatever the lengths of
+// its neighbours): lanes take ranks t0 + lane and walk the row's elements
+// in order, FEAT_U loads in flight (feature_sums).
+__global__ __launch_bounds__(WAVE) void fold_features_short_kernel(FoldArgs a, float* __restrict__ F) {
+    const int lane = threadIdx.x;
+    const uint32_t k = a.k;
+    const uint32_t row = blockIdx.x;
     const uint32_t s = a.ptr[row], e = a.ptr[row + 1], d = e - s;
     if (d > FOLD_SHORT) return;
     float* f = F + (size_t)row * (2 * k + 2);
@@ -317,8 +334,8 @@ hipError_t launch_fold_features(const FoldArgs& a, const uint32_t* long_rows, ui
                                 hipStream_t s) {
     if (a.rows == 0) return hipSuccess;
     if (a.k > (uint32_t)FOLD_KMAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fold_features_kernel, dim3(grid_rows(a.rows, nlong)), dim3(LBLOCK), 0, s, a, long_rows, nlong,
-                       F);
+    if (nlong) hipLaunchKernelGGL(fold_features_long_kernel, dim3(nlong), dim3(LBLOCK), 0, s, a, long_rows, F);
+    hipLaunchKernelGGL(fold_features_short_kernel, dim3(a.rows), dim3(WAVE), 0, s, a, F);
     return hipGetLastError();
 }
 

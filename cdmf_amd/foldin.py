@@ -122,7 +122,8 @@ class MLP:
         self._check(self._lib.ccd_mlp_set_norm(self._m, N.ptr(a, C.c_float), N.ptr(b, C.c_float),
                                                N.ptr(c, C.c_float), N.ptr(d, C.c_float)))
 
-    def forward(self, X, bf16: bool = False) -> np.ndarray:
+    def forward(self, X, bf16: int = 0) -> np.ndarray:
+        """bf16: 0 FP32, 1 bfloat16 weights, 2 matrix cores (ccd_mlp_forward)."""
         X = _f32(X)
         Y = np.empty((X.shape[0], self.dims[2]), np.float32)
         self._check(self._lib.ccd_mlp_forward(self._m, X.shape[0], N.ptr(X, C.c_float), N.ptr(Y, C.c_float), int(bf16)))
@@ -201,7 +202,9 @@ class FoldIn:
                                                     N.ptr(W, C.c_float), C.byref(s)))
         return W, s.value
 
-    def warm(self, mlp: MLP, ptr, idx, val, sweeps: int, bf16: bool = True):
+    def warm(self, mlp: MLP, ptr, idx, val, sweeps: int, bf16: int = 1):
+        """Warm start then `sweeps` sweeps; bf16 is the MLP inference mode
+        (0 FP32, 1 bfloat16 weights, 2 matrix cores; ccd_mlp_forward)."""
         ptr, idx, val = _u32(ptr), _u32(idx), _f32(val)
         W = np.empty((ptr.size - 1, self.k), np.float32)
         s = C.c_double()
@@ -284,17 +287,45 @@ class FoldinResult:
     warm_sweeps_obj: int = -1
     train_loss: float = float("nan")
     timings: dict = field(default_factory=dict)
+    # converged target: the test RMSE at the fold-in's fixed point (after
+    # `converge` sweeps; both starts reach the same one), and the fewest
+    # sweeps after which each start stays within rmse_tol of it
+    rmse_star: float = float("nan")
+    rmse_star_warm: float = float("nan")
+    rmse_tol: float = float("nan")
+    plain_sweeps_conv: int = -1
+    warm_sweeps_conv: int = -1
+    # the same for several tolerances: {tol: (plain sweeps, warm sweeps)}
+    conv_by_tol: dict = field(default_factory=dict)
+
+
+def _sweeps_to_stay_within(curve, star, tol) -> int:
+    """Fewest s with |curve[s'] - star| <= tol for every s' >= s (-1: never)."""
+    out = -1
+    for s in range(len(curve) - 1, -1, -1):
+        if abs(curve[s] - star) <= tol:
+            out = s
+        else:
+            break
+    return out
 
 
 def experiment(ds: Dataset, k: int, lam: float, outer: int, inner: int, sweeps: int = 8, epochs: int = 20,
                batch: int = 1024, lr: float = 1e-3, hidden: int = 256, seed: int = 5, converge: int = 40,
-               obj_rtol: float = 1e-3, log=None) -> FoldinResult:
+               obj_rtol: float = 1e-3, rmse_tol: float = 1e-4, max_sweeps: int = 0, mlp_mode: int = 2,
+               log=None) -> FoldinResult:
     """Held-out-users study: factorize the kept users, train the MLP on
     their (features, w_i) pairs, fold in the held-out users from zero and
-    from the warm start.  Two targets: the plain fold-in's test RMSE after
+    from the warm start.  Three targets: the plain fold-in's test RMSE after
     `sweeps` sweeps (the warm start's count is the fewest sweeps that reach
-    it), and the fold-in objective within obj_rtol of its value after
-    `converge` plain sweeps (fewest sweeps for each start)."""
+    it); the fold-in objective within obj_rtol of its value after `converge`
+    plain sweeps; and the converged test RMSE (after `converge` sweeps from
+    either start: per-row fold-in is a convex ridge problem, so both reach
+    the same fixed point), for which each start's count is the fewest sweeps
+    after which it stays within rmse_tol of it.  Curves are recorded up to
+    max(sweeps, max_sweeps) sweeps.  mlp_mode: MLP inference of the warm
+    start (1: bfloat16 weights, FP32 arithmetic; 2: the matrix-core path,
+    see ccd_mlp_forward)."""
     say = log or (lambda *_: None)
     t = time.perf_counter()
     sp = split_users(ds, 0.1, seed)
@@ -322,21 +353,40 @@ def experiment(ds: Dataset, k: int, lam: float, outer: int, inner: int, sweeps: 
                                                                      res.timings["train_s"]))
     args = (sp.new_ptr, sp.new_idx, sp.new_val)
     test = (sp.new_test_row, sp.new_test_col, sp.new_test_val)
-    fold.warm(mlp, *args, sweeps=0, bf16=True)  # first call: buffers, bf16 weights
+    fold.warm(mlp, *args, sweeps=0, bf16=mlp_mode)  # first call: buffers, bf16 weights
     Wc, _ = fold.rows(*args, sweeps=converge)
     res.obj_star = fold_objective(Wc, H, *args, lam)
-    for sw in range(sweeps + 1):
+    res.rmse_star = fold_rmse(Wc, H, *test)
+    Wcw, _ = fold.warm(mlp, *args, sweeps=converge, bf16=mlp_mode)
+    res.rmse_star_warm = fold_rmse(Wcw, H, *test)
+    res.rmse_tol = rmse_tol
+    say("fixed point after %d sweeps: test RMSE %.6f from zero, %.6f from the warm start" % (
+        converge, res.rmse_star, res.rmse_star_warm))
+    for sw in range(max(sweeps, max_sweeps) + 1):
         Wp, tp = fold.rows(*args, sweeps=sw)
         res.plain_rmse.append(fold_rmse(Wp, H, *test))
-        res.plain_obj.append(fold_objective(Wp, H, *args, lam))
         res.plain_time.append(tp)
-        Ww, tw = fold.warm(mlp, *args, sweeps=sw, bf16=True)
+        Ww, tw = fold.warm(mlp, *args, sweeps=sw, bf16=mlp_mode)
         res.warm_rmse.append(fold_rmse(Ww, H, *test))
-        res.warm_obj.append(fold_objective(Ww, H, *args, lam))
         res.warm_time.append(tw)
-        say("sweeps %d: plain %.5f obj %.6g (%.2f ms)  warm %.5f obj %.6g (%.2f ms)" % (
-            sw, res.plain_rmse[-1], res.plain_obj[-1] / res.obj_star - 1, 1e3 * tp, res.warm_rmse[-1],
-            res.warm_obj[-1] / res.obj_star - 1, 1e3 * tw))
+        if sw <= sweeps:  # the objective (a pass over every rating on the host) for the first sweeps only
+            res.plain_obj.append(fold_objective(Wp, H, *args, lam))
+            res.warm_obj.append(fold_objective(Ww, H, *args, lam))
+            say("sweeps %d: plain %.5f obj %.6g (%.2f ms)  warm %.5f obj %.6g (%.2f ms)" % (
+                sw, res.plain_rmse[-1], res.plain_obj[-1] / res.obj_star - 1, 1e3 * tp, res.warm_rmse[-1],
+                res.warm_obj[-1] / res.obj_star - 1, 1e3 * tw))
+        else:
+            say("sweeps %d: plain %.6f (%.2f ms)  warm %.6f (%.2f ms)" % (
+                sw, res.plain_rmse[-1], 1e3 * tp, res.warm_rmse[-1], 1e3 * tw))
+    res.plain_sweeps_conv = _sweeps_to_stay_within(res.plain_rmse, res.rmse_star, rmse_tol)
+    res.warm_sweeps_conv = _sweeps_to_stay_within(res.warm_rmse, res.rmse_star, rmse_tol)
+    for tol in sorted({1e-3, 5e-4, 2e-4, 1e-4, rmse_tol}, reverse=True):
+        res.conv_by_tol[tol] = (_sweeps_to_stay_within(res.plain_rmse, res.rmse_star, tol),
+                                _sweeps_to_stay_within(res.warm_rmse, res.rmse_star, tol))
+        say("within %.0e of the converged test RMSE from: plain %d sweeps, warm %d sweeps" % (
+            tol, *res.conv_by_tol[tol]))
+    say("within %.0e of the converged test RMSE from: plain %d sweeps, warm %d sweeps" % (
+        rmse_tol, res.plain_sweeps_conv, res.warm_sweeps_conv))
     res.target = res.plain_rmse[sweeps]
     res.plain_sweeps = sweeps
     res.warm_sweeps = next((sw for sw, v in enumerate(res.warm_rmse) if v <= res.target), -1)

@@ -45,7 +45,7 @@ class CPlan(C.Structure):
 
 
 class CBatchPlan(C.Structure):
-    _fields_ = [("sigma", C.c_uint32), ("blocks", C.c_uint32), ("work_exponent", C.c_float), ("reserved", C.c_uint32)]
+    _fields_ = [("sigma", C.c_uint32), ("blocks", C.c_uint32), ("work_exponent", C.c_float), ("solo", C.c_uint32)]
 
 
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double)
@@ -89,6 +89,7 @@ def hip_lib() -> C.CDLL:
             "ccd_batch_run": ([vp, C.c_int32], C.c_int32),
             "ccd_batch_time": ([vp, f64p], C.c_int32),
             "ccd_batch_teams": ([vp, u32p, C.c_int32], C.c_int32),
+            "ccd_batch_tune_times": ([vp, f64p, C.c_int32], C.c_int32),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -105,6 +106,7 @@ ABI_SYMBOLS = (
     "ccd_rmse", "ccd_objective", "ccd_iter_rmse", "ccd_kernel_stats", "ccd_loop_time", "ccd_copy_probe", "ccd_set_plan",
     "ccd_batch_create", "ccd_batch_destroy", "ccd_batch_last_error", "ccd_batch_add", "ccd_batch_set_plan",
     "ccd_batch_get_plan", "ccd_batch_tune", "ccd_batch_run", "ccd_batch_time", "ccd_batch_teams",
+    "ccd_batch_tune_times",
 )
 
 
@@ -345,13 +347,15 @@ class Batch:
         self.solvers.append(solver)
         return idx
 
-    def set_plan(self, sigma: int = 16, blocks: int = 0, work_exponent: float = 1.0) -> None:
-        self._check(self._lib.ccd_batch_set_plan(self._b, C.byref(CBatchPlan(sigma, blocks, work_exponent, 0))))
+    def set_plan(self, sigma: int = 16, blocks: int = 0, work_exponent: float = 1.0, solo: int = 0) -> None:
+        """solo: the largest problems solved alone on the whole GPU first
+        (ccd_batch_plan.solo)."""
+        self._check(self._lib.ccd_batch_set_plan(self._b, C.byref(CBatchPlan(sigma, blocks, work_exponent, solo))))
 
     def plan(self) -> dict:
         pp = CBatchPlan()
         self._check(self._lib.ccd_batch_get_plan(self._b, C.byref(pp)))
-        return dict(sigma=pp.sigma, blocks=pp.blocks, work_exponent=pp.work_exponent)
+        return dict(sigma=pp.sigma, blocks=pp.blocks, work_exponent=pp.work_exponent, solo=pp.solo)
 
     def tune(self) -> float:
         ms = C.c_double()
@@ -359,11 +363,18 @@ class Batch:
         return ms.value
 
     def run(self, use_graph: bool = False) -> float:
-        """One run of every problem; returns its device time in seconds."""
+        """One run of every problem; returns its time in seconds
+        (ccd_batch_time)."""
         self._check(self._lib.ccd_batch_run(self._b, int(use_graph)))
         s = C.c_double()
         self._check(self._lib.ccd_batch_time(self._b, C.byref(s)))
         return s.value
+
+    def tune_times(self) -> list:
+        """Whole-run times (ms) of the last tune for solo = 0, 1, ... (ccd_batch_tune_times)."""
+        buf = np.zeros(16)
+        n = self._check(self._lib.ccd_batch_tune_times(self._b, N.ptr(buf, C.c_double), buf.size))
+        return [float(x) for x in buf[:min(n, buf.size)]]
 
     def teams(self) -> list:
         buf = np.zeros(max(len(self.solvers), 1), dtype=np.uint32)

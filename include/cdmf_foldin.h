@@ -26,8 +26,12 @@ extern "C" {
  * flat FP32: W1 [hidden][in], b1 [hidden], W2 [hidden][hidden], b2 [hidden],
  * W3 [out][hidden], b3 [out].  Inputs are standardised with in_mean/in_std
  * and outputs mapped back with out_mean/out_std (ccd_mlp_set_norm; identity
- * until set).  Trained in FP32 with Adam; inference reads the weights as
- * FP32 or as bfloat16 (round to nearest even), arithmetic in FP32. */
+ * until set).  Trained in FP32 with Adam.  Inference (bf16_weights below):
+ * 0 = FP32; 1 = weights read as bfloat16 (round to nearest even),
+ * arithmetic in FP32; 2 = the matrix-core path: one launch on the bfloat16
+ * MFMA units, inputs, weights and hidden activations rounded to bfloat16,
+ * products accumulated in FP32 (needs 256 hidden units, in <= 288,
+ * out <= 256). */
 typedef struct ccd_mlp ccd_mlp;
 
 int32_t ccd_mlp_create(int32_t device, uint32_t in, uint32_t hidden, uint32_t out, uint64_t seed, ccd_mlp** m);
@@ -38,7 +42,8 @@ int32_t ccd_mlp_get_params(ccd_mlp* m, float* params);
 int32_t ccd_mlp_set_params(ccd_mlp* m, const float* params);
 int32_t ccd_mlp_set_norm(ccd_mlp* m, const float* in_mean, const float* in_std, const float* out_mean,
                          const float* out_std);
-/* Y [rows][out] = MLP(X [rows][in]) (raw features in, raw outputs out). */
+/* Y [rows][out] = MLP(X [rows][in]) (raw features in, raw outputs out);
+ * bf16_weights 0, 1 or 2 as above. */
 int32_t ccd_mlp_forward(ccd_mlp* m, uint64_t rows, const float* X, float* Y, int32_t bf16_weights);
 /* Loss mean((MLP_n(Xn) - Tn)^2) and its gradient (flat, like the
  * parameters) on already standardised inputs Xn and targets Tn, FP32
@@ -76,7 +81,8 @@ int32_t ccd_fold_rows(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint
  * (from several threads or fold objects) race. */
 int32_t ccd_fold_rows_noalloc(ccd_fold* f, int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
                               const float* W0, int32_t sweeps, float* W, double* seconds);
-/* Warm start: features -> MLP (bfloat16 weights if bf16_weights) -> W0,
+/* Warm start: features -> MLP (bf16_weights 0, 1 or 2 as for
+ * ccd_mlp_forward) -> W0,
  * then `sweeps` sweeps; the MLP must map 2k + 2 inputs to k outputs.
  * *seconds: device time of the whole pipeline. */
 int32_t ccd_fold_warm(ccd_fold* f, ccd_mlp* m, int64_t rows, const uint32_t* ptr, const uint32_t* idx,

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CCD_ABI_VERSION 4
+#define CCD_ABI_VERSION 5
 
 /* Status codes: 0 = success, negative = error (message: ccd_last_error). */
 #define CCD_OK 0
@@ -223,7 +223,12 @@ typedef struct ccd_batch_plan {
     uint32_t sigma;       /* elements per lane of a tile: 8 or 16 (default 16) */
     uint32_t blocks;      /* persistent blocks of 256 threads, a multiple of 8; 0 = all co-resident ones */
     float work_exponent;  /* blocks are dealt to problems by (their work)^work_exponent (default 1) */
-    uint32_t reserved;
+    uint32_t solo;        /* the `solo` problems of largest work (k * maxiter * (2 maxinneriter + 2) * nnz)
+                             are solved first, one after another, each on the whole GPU (ccd_run of the
+                             tiled sweeps, plain tile layout, same sigma: the same result); the persistent
+                             launch takes the rest (default 0, at most 64).  A team of the launch is
+                             confined to one XCD, so a problem with more than an eighth of the work bounds
+                             the launch. */
 } ccd_batch_plan;
 
 int32_t ccd_batch_create(int32_t device, ccd_batch** out);
@@ -235,7 +240,11 @@ int32_t ccd_batch_set_plan(ccd_batch* b, const ccd_batch_plan* pp);
 int32_t ccd_batch_get_plan(ccd_batch* b, ccd_batch_plan* out);
 /* Autotuner: times short trial launches (about 1/64 of each problem's
  * sweeps, no state change) for every sigma and work exponent and keeps the
- * fastest plan; *best_ms = that trial's time. */
+ * fastest; then, when no problem continues an earlier run, times whole runs
+ * with 0 to 3 problems solved alone (solo), putting every problem's factors
+ * and residual back to their start after each, and keeps the fastest.
+ * *best_ms = the kept plan's trial time, or its whole-run time when solo
+ * counts were timed. */
 int32_t ccd_batch_tune(ccd_batch* b, double* best_ms);
 /* One run of every problem: set-up fills and retiling, the persistent
  * launch, write-back.  use_graph: capture that sequence in a HIP graph on
@@ -243,9 +252,14 @@ int32_t ccd_batch_tune(ccd_batch* b, double* best_ms);
  * zero_first state and their device buffers stay the same (a re-upload, a
  * new k or a ccd_run that rebuilds a tile layout recaptures it). */
 int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph);
-/* device time of the last run (events around the whole sequence) */
+/* time of the last run: host time of the problems solved alone plus the
+ * device time of the launch sequence (events around it) */
 int32_t ccd_batch_time(ccd_batch* b, double* seconds);
-/* blocks given to each problem by the last run; returns the problem count */
+/* Whole-run times (ms) the last ccd_batch_tune measured for solo = 0, 1, ...
+ * (none when it timed trials only); returns how many there are. */
+int32_t ccd_batch_tune_times(ccd_batch* b, double* ms, int32_t max_count);
+/* blocks given to each problem by the last run (0: solved alone); returns
+ * the problem count */
 int32_t ccd_batch_teams(ccd_batch* b, uint32_t* blocks, int32_t max_count);
 
 #ifdef __cplusplus

@@ -206,3 +206,25 @@ def test_batch_graph_after_layout_rebuild():
     for (name, *_), e, g in zip(cases, out[False], out[True]):
         assert all(np.all(np.isfinite(x)) for x in e), name
         _same(g, e, name + " (graph after a layout rebuild)")
+
+
+@pytest.mark.parametrize("solo", [1, 2])
+def test_batch_solo_problems_bitwise(solo):
+    """ccd_batch_plan.solo: the largest problems are solved alone on the
+    whole GPU before the persistent launch takes the rest; every problem's
+    result is still bit-identical to its own single-problem run, for the
+    eager and the graph-replayed launch."""
+    S = _solver()
+    cases = _cases()
+    for graph in (False, True):
+        b, solvers = _build(S, cases, 16)
+        b.set_plan(sigma=16, solo=solo)
+        assert b.plan()["solo"] == solo
+        t = b.run(use_graph=graph)
+        teams = b.teams()
+        _say("solo=%d graph=%s: %.3f ms, teams %s" % (solo, graph, 1e3 * t, teams))
+        assert sum(1 for x in teams if x == 0) == solo
+        for (name, ds, k, lam, outer, inner, order, W0, H0), g in zip(cases, _batch_results(solvers)):
+            _same(g, _single(S, ds, k, lam, outer, inner, order, W0, H0, 16), "%s (solo=%d, graph=%s)" % (
+                name, solo, graph))
+        b.close()
