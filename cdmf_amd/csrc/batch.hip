@@ -110,7 +110,7 @@ int prepare(ccd_batch* b, int sigma) {
         ccd_ctx* c = b->items[i].ctx;
         c->prec = CCD_PREC_FP32;
         for (ccdi::DevLayout* L : {&c->csr, &c->csc}) {
-            if (int rc = ccdi::build_tiled(c, *L, sigma, false)) return ctx_err(b, i, rc);
+            if (int rc = ccdi::build_tiled(c, *L, sigma, 0u)) return ctx_err(b, i, rc);
             if (!L->pair) {
                 double2* d = nullptr;
                 if (ccdi::dalloc(&d, ((size_t)L->gdim + 1) / 2) != hipSuccess)

@@ -285,6 +285,15 @@ Hot hot_prefix(const ccd_ctx* c, DevLayout& L, size_t budget, bool fused, bool s
     return {L.split_x, L.split_p};
 }
 
+// Split bound of the tiled sweeps with the plan's split_rows: the LDS prefix
+// of a plain FP32 sweep with one block per CU (FP32 entries of the LDS a
+// block may use, 1 KiB spare, as launch_shape sizes it); rows of a layout
+// whose gathered vector exceeds it are split there.  0: no split.
+uint32_t split_bound(const ccd_ctx* c) {
+    if (c->plan.split_rows == 0) return 0;
+    return (uint32_t)(((size_t)std::max(0, std::min(c->lds_block, c->lds_cu) - 1024) & ~(size_t)255) / 4);
+}
+
 int do_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* xs, const void* ys, const void* xa,
             const void* ya, bool x_is_cur, bool sub, bool add, bool sweep, float lambda, int kind) {
     const bool fused = sweep && (sub || add);
@@ -363,18 +372,32 @@ int upload_vec(ccd_ctx* c, T** dst, const std::vector<T>& v) {
 
 // Tile descriptors from the row pointer (host, tile_layout.cpp), index array
 // tiled on the device.
-// split: with rows split at L.split_at, keep each row's two parts as two
-// virtual rows, all first parts (rows in order) before all second parts,
-// so that a tile gathers either from the LDS prefix only or from beyond it
-// only (the batched kernel asks for the plain layout).
-int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
-    split = split && L.split_at != 0;
-    if (L.tl.sigma == sigma && L.tl.split == split) return CCD_OK;
+// split_at > 0: split each row at internal index split_at (its elements
+// ascend by index) into two virtual rows, all first parts (rows in order)
+// before all second parts, so that a tile gathers either from below the
+// bound only or from above it only; 0, or a bound past the gathered
+// vector: one tile range (the batched kernel asks for it).
+int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, uint32_t split_at) {
+    if ((int64_t)split_at >= L.gdim) split_at = 0;
+    const bool split = split_at != 0;
+    if (L.tl.sigma == sigma && L.split_at == split_at) return CCD_OK;
     // a captured outer iteration (or batch) holds the old layout's pointers
     drop_graph(c);
     ++c->gen;
     free_tiled(L.tl);
+    L.split_at = 0;
+    L.below.clear();
     TiledLayout& T = L.tl;
+    if (split) {
+        uint32_t* d = nullptr;
+        HIP_TRY(c, dalloc(&d, (size_t)std::max<int64_t>(L.rows, 1)));
+        L.below.resize((size_t)L.rows);
+        hipError_t e = ccd::launch_count_below(L.ptr, L.idx, L.idx16, (uint64_t)L.rows, split_at, d, c->stream);
+        if (e == hipSuccess && L.rows)
+            e = ccd::copy_sync(c->stream, L.below.data(), d, sizeof(uint32_t) * (size_t)L.rows, hipMemcpyDeviceToHost);
+        dfree(d);
+        HIP_TRY(c, e);
+    }
     ccd::TileTables H;
     const std::string why =
         ccd::build_tile_tables(L.hptr.data(), L.rows, L.nnz, split ? L.below.data() : nullptr, sigma, split, H);
@@ -414,6 +437,7 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     T.sigma = sigma;  // only a complete layout is reused
     T.split = split;
+    L.split_at = split_at;
     return CCD_OK;
 }
 
@@ -423,10 +447,14 @@ int do_tiled_pass(ccd_ctx* c, DevLayout& L, const void* x, void* y, const void* 
     const bool fused = sweep && (sub || add);
     const Launch sh = launch_shape(c, true, T.sigma, T.ib, sub, add, sweep, fused && x_is_cur);
     const Hot hot = hot_prefix(c, L, sh.lds_budget, fused, sweep, x_is_cur);
+    // tiles of second parts gather from beyond the LDS prefix only when the
+    // split is at or past it (a prefix staged with two blocks per CU, or in
+    // bfloat16, differs from the one the layout was split at)
+    const uint32_t cold0 = L.split_at >= hot.x ? T.cold_tile0 : 0xffffffffu;
     ccd::TiledArgs a{L.ptr, T.tidx, T.tval, sweep ? x : nullptr, y, xs, ys, xa, ya, fused ? L.pair : nullptr,
                      T.flags, T.row0, T.segoff, T.segrows, T.open, T.gmax, T.head, T.tail, L.nnz, T.ntiles,
                      hot.x, hot.p, (uint32_t)(fused && x_is_cur), lambda, T.cold_row, T.nreal,
-                     sweep ? T.part : nullptr, T.cold_tile0};
+                     sweep ? T.part : nullptr, cold0, T.cold_tile0};
     {
         Timed tm(c, kind, true);
         HIP_TRY(c, ccd::launch_tiled(a, T.sigma, c->prec, T.ib, sub, add, sweep, sh.blocks, c->stream, tm.ev()));
@@ -452,29 +480,25 @@ std::vector<uint32_t> degree_order(const uint32_t* ptr, int64_t rows) {
     return perm;
 }
 
-// Element order inside each row of a renumbered layout: the caller's, sorted
-// by internal index, or split into the elements whose internal index is
-// below a bound, in the caller's order, followed by the others sorted by
-// internal index (their gathers miss the LDS prefix; sorted, a lane's
-// consecutive elements share cache lines).
-enum class RowOrder { keep, sort, split };
+// Element order inside each row of a renumbered layout: the caller's, or
+// ascending internal index (so that a row splits at any bound into the
+// elements gathered from below it and the rest, and a lane's consecutive
+// elements beyond the LDS prefix share cache lines).
+enum class RowOrder { keep, sort };
 
 // One compressed layout in internal numbering: rows in `perm` order, indices
 // mapped through `ipos` (caller's index -> internal), elements inside each
-// row in `order` (split at internal index `at`; equal sort keys keep their
-// order).  src[o]: the caller's position of internal element o.  below[i]
-// (split only): elements of row i below `at`.  Rows are processed in
-// parallel, in chunks taken from a shared counter.
+// row in `order` (equal sort keys keep their order).  src[o]: the caller's
+// position of internal element o.  Rows are processed in parallel, in
+// chunks taken from a shared counter.
 void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const float* val,
               const std::vector<uint32_t>& perm, const std::vector<uint32_t>& ipos, std::vector<uint32_t>& optr,
-              std::vector<uint32_t>& oidx, std::vector<float>& oval, std::vector<uint32_t>& src, RowOrder order,
-              uint32_t at, std::vector<uint32_t>& below) {
+              std::vector<uint32_t>& oidx, std::vector<float>& oval, std::vector<uint32_t>& src, RowOrder order) {
     optr.assign((size_t)rows + 1, 0u);
     for (int64_t i = 0; i < rows; ++i) optr[i + 1] = optr[i] + (ptr[perm[i] + 1] - ptr[perm[i]]);
     oidx.resize(optr[rows]);
     oval.resize(optr[rows]);
     src.resize(optr[rows]);
-    below.assign(order == RowOrder::split ? (size_t)rows : 0, 0u);
     constexpr int64_t CHUNK = 512;
     std::atomic<int64_t> next{0};
     auto work = [&]() {
@@ -483,14 +507,9 @@ void renumber(int64_t rows, const uint32_t* ptr, const uint32_t* idx, const floa
             for (int64_t i = b; i < std::min(rows, b + CHUNK); ++i) {
                 const uint32_t r = perm[i], e0 = ptr[r], len = ptr[r + 1] - ptr[r];
                 key.resize(len);
-                uint32_t nb = 0;
-                for (uint32_t q = 0; q < len; ++q) {
-                    const uint32_t k = ipos[idx[e0 + q]];
-                    nb += k < at ? 1u : 0u;
-                    key[q] = (uint64_t)(order == RowOrder::sort || k >= at ? k : 0u) << 32 | q;
-                }
+                for (uint32_t q = 0; q < len; ++q)
+                    key[q] = (uint64_t)(order == RowOrder::sort ? ipos[idx[e0 + q]] : 0u) << 32 | q;
                 if (order != RowOrder::keep) std::sort(key.begin(), key.end());
-                if (order == RowOrder::split) below[i] = nb;
                 const uint32_t o = optr[i];
                 for (uint32_t q = 0; q < len; ++q) {
                     const uint32_t e = e0 + (uint32_t)key[q];
@@ -649,27 +668,17 @@ int32_t ccd_upload(ccd_ctx* c, int64_t m, int64_t n, uint64_t nnz, const uint32_
         for (int64_t j = 0; j < n; ++j) cpos[cperm[j]] = (uint32_t)j;
         std::vector<uint32_t> p2, i2, src;
         std::vector<float> v2;
-        // The LDS prefix of a plain sweep (FP32 entries of one block per CU,
-        // as launch_shape sizes it).  A layout whose gathered vector fits it
-        // gets its rows sorted by gathered index; a larger one gets each row
-        // split at it, so the tiled sweeps can keep the elements gathered
-        // from beyond the prefix in tiles of their own (build_tiled).
-        const uint32_t lds_entries =
-            (uint32_t)(((size_t)std::max(0, std::min(c->lds_block, c->lds_cu) - 1024) & ~(size_t)255) / 4);
-        std::vector<uint32_t> below;
-        auto order_for = [&](int64_t gdim) { return gdim <= lds_entries ? RowOrder::sort : RowOrder::split; };
-        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2, src, order_for(n), lds_entries, below);
+        // Rows sorted by gathered index: the tiled sweeps can then split
+        // each row at any bound (build_tiled) and keep the elements gathered
+        // from beyond an LDS prefix in tiles of their own.
+        renumber(m, row_ptr, col_idx, csr_val, rperm, cpos, p2, i2, v2, src, RowOrder::sort);
         int rc = upload_layout(c, c->csr, m, n, nnz, p2.data(), i2.data(), v2.data());
         if (rc) return rc;
         c->csr.src = std::move(src);
-        c->csr.below = std::move(below);
-        c->csr.split_at = order_for(n) == RowOrder::split ? lds_entries : 0u;
-        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2, src, order_for(m), lds_entries, below);
+        renumber(n, col_ptr, row_idx, csc_val, cperm, rpos, p2, i2, v2, src, RowOrder::sort);
         rc = upload_layout(c, c->csc, n, m, nnz, p2.data(), i2.data(), v2.data());
         if (rc) return rc;
         c->csc.src = std::move(src);
-        c->csc.below = std::move(below);
-        c->csc.split_at = order_for(m) == RowOrder::split ? lds_entries : 0u;
         c->csr.perm = std::move(rperm);
         c->csc.perm = std::move(cperm);
     }
@@ -821,7 +830,13 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
         c->kind_n[q] = 0;
         c->kind_seen[q] = 0;
     }
-    const bool tl = p->variant == CCD_VARIANT_TILED, sp = c->plan.split_rows != 0;
+    const bool tiled = p->variant == CCD_VARIANT_TILED;
+    if (tiled) {
+        int rc0 = build_tiled(c, F, (int)c->plan.sigma, split_bound(c));
+        if (!rc0) rc0 = build_tiled(c, S, (int)c->plan.sigma, split_bound(c));
+        if (rc0) return rc0;
+    }
+    const bool tl = tiled, sp = c->plan.split_rows != 0;
     c->kind_bytes[CCD_K_SWEEP_F] = layout_bytes(F, tl, sp, false, 1, 0, true);
     c->kind_bytes[CCD_K_SWEEP_S] = layout_bytes(S, tl, sp, false, 1, 0, true);
     c->kind_bytes[CCD_K_FUSED_F] = layout_bytes(F, tl, sp, true, 2, 2, true);
@@ -832,12 +847,8 @@ int32_t ccd_run(ccd_ctx* c, const ccd_params* p, ccd_iter_cb cb, void* user) {
     c->kind_bytes[CCD_K_RMSE] = 12.0 * (double)c->nnz_test + 4.0 * (double)k * (double)(c->m + c->n);
     c->kind_bytes[CCD_K_COPY] = 5.0 * (p->precision == CCD_PREC_BF16 ? 2.0 : 4.0) * (double)(c->m + c->n);
 
-    const bool tiled = p->variant == CCD_VARIANT_TILED;
     const PassFn pass = tiled ? do_tiled_pass : do_pass;
     if (tiled) {
-        int rc0 = build_tiled(c, F, (int)c->plan.sigma, c->plan.split_rows != 0);
-        if (!rc0) rc0 = build_tiled(c, S, (int)c->plan.sigma, c->plan.split_rows != 0);
-        if (rc0) return rc0;
         // residual into tile order; rows without entries keep y = 0
         if (F.tl.ntiles)
             HIP_TRY(c, ccd::launch_retile_f32(F.val, F.tl.tval, F.nnz, F.tl.ntiles, F.tl.sigma, true, c->stream, F.tl.vpos));

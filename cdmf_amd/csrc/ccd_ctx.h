@@ -74,8 +74,9 @@ struct DevLayout {
     uint32_t split_x = 0, split_p = 0;
     std::vector<uint32_t> hptr;      // host copy of the internal ptr (planning)
     std::vector<uint32_t> src;       // caller's position of each internal element (residual download)
-    // Rows split at internal index split_at (0: rows sorted, not split):
-    // below[i] leading elements of row i are gathered from below it.
+    // Split of the current tiled layout (build_tiled): rows, whose elements
+    // ascend by internal index, split at split_at (0: not split); below[i]
+    // leading elements of row i are gathered from below it.
     uint32_t split_at = 0;
     std::vector<uint32_t> below;
     std::vector<uint32_t> perm;      // internal row -> caller's row
@@ -166,6 +167,6 @@ void drop_graph(ccd_ctx* c);
 // up to 2^24, else 4.
 int tiled_index_bytes(const DevLayout& L);
 // Tile descriptors of layout L for tiles of 64 * sigma elements (cached).
-int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, bool split);
+int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, uint32_t split_at);
 
 }  // namespace ccdi

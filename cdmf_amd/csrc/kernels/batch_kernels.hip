@@ -104,6 +104,7 @@ __device__ __forceinline__ TiledArgs layout_args(const BatchLayout& L, const Pas
     a.cold_row = nullptr;   // the batched kernel works on plain layouts
     a.nreal = 0;
     a.cold_tile0 = 0xffffffffu;
+    a.spread_tile0 = 0xffffffffu;
     a.part = nullptr;
     a.head = L.head;
     a.tail = L.tail;

@@ -279,6 +279,10 @@ hipError_t launch_to_bf16(const float* src, bf16* dst, uint64_t count, hipStream
 hipError_t launch_from_bf16(const bf16* src, float* dst, uint64_t count, hipStream_t s);
 
 hipError_t launch_copy(const float4* src, float4* dst, uint64_t count, hipStream_t s);
+// out[r] = elements of row r (of a layout whose rows ascend by index) with
+// index below `bound`
+hipError_t launch_count_below(const uint32_t* ptr, const void* idx, bool idx16, uint64_t rows, uint32_t bound,
+                              uint32_t* out, hipStream_t s);
 
 // ---------------------------------------------------------------- tiled ---
 // Tiled layout of one compressed matrix (the CSR5 idea of
@@ -335,6 +339,9 @@ struct TiledArgs {
     // First tile of a split layout whose elements all belong to second parts
     // (every gather misses the LDS prefix); 0xffffffff: none.
     uint32_t cold_tile0;
+    // First tile of the layout's costly tail (second parts of split rows),
+    // whose tiles a block spreads over its whole pass; 0xffffffff: none.
+    uint32_t spread_tile0;
 };
 
 // spans[0, nshort) cover two tiles (t1 = t0 + 1) and are finished one per

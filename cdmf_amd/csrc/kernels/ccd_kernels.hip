@@ -369,6 +369,25 @@ __global__ __launch_bounds__(BLOCK) void from_bf16_kernel(const bf16* __restrict
         dst[i] = __uint_as_float((uint32_t)src[i] << 16);
 }
 
+// Elements of each row below `bound`, for rows whose indices ascend: a
+// binary search per row (one thread each).
+template <typename IT>
+__global__ __launch_bounds__(BLOCK) void count_below_kernel(const uint32_t* __restrict__ ptr,
+                                                            const IT* __restrict__ idx, uint64_t rows,
+                                                            uint32_t bound, uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t r = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; r < rows; r += stride) {
+        uint32_t lo = ptr[r], hi = ptr[r + 1];
+        const uint32_t b = lo;
+        while (lo < hi) {  // first position whose index is >= bound
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if ((uint32_t)idx[mid] < bound) lo = mid + 1;
+            else hi = mid;
+        }
+        out[r] = lo - b;
+    }
+}
+
 template <typename TF>
 __global__ __launch_bounds__(BLOCK) void commit_kernel(CommitArgs a) {
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -500,6 +519,18 @@ hipError_t launch_commit(const CommitArgs& a, bool bf16_factors, hipStream_t s) 
     if (count == 0) return hipSuccess;
     if (bf16_factors) hipLaunchKernelGGL(commit_kernel<bf16>, dim3(grid_for(count)), dim3(BLOCK), 0, s, a);
     else hipLaunchKernelGGL(commit_kernel<float>, dim3(grid_for(count)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_below(const uint32_t* ptr, const void* idx, bool idx16, uint64_t rows, uint32_t bound,
+                              uint32_t* out, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    if (idx16)
+        hipLaunchKernelGGL(count_below_kernel<uint16_t>, dim3(grid_for(rows)), dim3(BLOCK), 0, s, ptr,
+                           static_cast<const uint16_t*>(idx), rows, bound, out);
+    else
+        hipLaunchKernelGGL(count_below_kernel<uint32_t>, dim3(grid_for(rows)), dim3(BLOCK), 0, s, ptr,
+                           static_cast<const uint32_t*>(idx), rows, bound, out);
     return hipGetLastError();
 }
 

@@ -409,13 +409,17 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
 
     // A sweep tile with gmax below the staged counts has consecutive rows
     // (tiles with empty rows carry 0xffffffff) and gathers only from LDS.
-    // The other tiles' rows are consecutive unless the tile holds empty
-    // rows.  Copies of the rest are chosen by wave-uniform branches, so the
-    // common one has no segment-table loads (a load on only one of two merged
-    // paths makes the compiler wait for every outstanding load at each later
-    // use), no global gathers and, for 3-byte indices, no high bytes.
-    // Plain sweeps: a tile of second parts only (consecutive non-empty rows,
-    // no padding) gathers straight from global memory.
+    // Copies for the common cases are chosen by wave-uniform branches, so
+    // the common one has no segment-table loads (a load on only one of two
+    // merged paths makes the compiler wait for every outstanding load at
+    // each later use), no global gathers and, for 3-byte indices, no high
+    // bytes.  Plain sweeps: a tile of second parts only (consecutive
+    // non-empty rows, no padding) gathers straight from global memory.
+    // Fused passes keep only the general copies (per-tile choice of row
+    // lookup, LDS or global gathers; with 3-byte indices, with or without
+    // the high bytes): every copy inlined costs scalar registers, and past
+    // the 102 a wave has, the compiler parks values in vector lanes and
+    // reads them back with a VALU op at each use.
     if constexpr (SPLIT && SWEEP && !SUB && !ADD) {
         if (tile >= a.cold_tile0 && gmax != 0xffffffffu) {
             load_tile<IdxT, SIGMA, false>(a, tile, lane, c, r, flags, seg0);
@@ -425,7 +429,7 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
             return;
         }
     }
-    if (SPLIT && SWEEP && gmax < hot_limit<IdxT, SUB, ADD, XCUR>(a)) {
+    if (SPLIT && SWEEP && !(SUB || ADD) && gmax < hot_limit<IdxT, SUB, ADD, XCUR>(a)) {
         if constexpr (EARLY) descriptors();
         load_tile<IdxT, SIGMA, EARLY, false>(a, tile, lane, c, r, flags, seg0);
         if constexpr (!EARLY) descriptors();
@@ -442,8 +446,8 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
             if constexpr (EARLY) descriptors();
             load_tile<IdxT, SIGMA, EARLY, false>(a, tile, lane, c, r, flags, seg0);
             if constexpr (!EARLY) descriptors();
-            tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN>(a, lds, ldsp, tile, lane, flags, seg0, row0,
-                                                                         segoff, open, c, r);
+            tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, (SUB || ADD) ? SEG_ANY : SEG_RUN>(
+                a, lds, ldsp, tile, lane, flags, seg0, row0, segoff, open, c, r);
             return;
         }
     }
@@ -453,12 +457,9 @@ __device__ __forceinline__ void tile_body(const TiledArgs& a, const TF* lds, con
     if constexpr (!SPLIT)
         tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_ANY, false, true>(a, lds, ldsp, tile, lane, flags,
                                                                                   seg0, row0, segoff, open, c, r);
-    else if (segoff == 0xffffffffu)
-        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_RUN>(a, lds, ldsp, tile, lane, flags, seg0, row0,
-                                                                     segoff, open, c, r);
     else
-        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_TABLE>(a, lds, ldsp, tile, lane, flags, seg0, row0,
-                                                                       segoff, open, c, r);
+        tile_core<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR, SEG_ANY>(a, lds, ldsp, tile, lane, flags, seg0, row0,
+                                                                     segoff, open, c, r);
 }
 
 // A row spanning two tiles, by one lane: tail[t0] + head[t0 + 1], rounded

@@ -9,7 +9,8 @@
 // ever needed.
 //
 // Persistent kernel: each 1024-thread block stages the hot prefixes of the
-// gathered vectors in LDS, then its waves take tiles in a grid-stride loop.
+// gathered vectors in LDS, then its waves take the block's tiles (every B-th
+// tile of the layout) one at a time from a counter in LDS.
 // One wave per tile of 64*SIGMA elements.  Lane l owns elements
 // l*SIGMA .. l*SIGMA+SIGMA-1 of the tile; they are stored in groups of
 // G = 16 / (bytes per entry), element i of lane l at (i/G)*64G + G*l + i%G,
@@ -45,24 +46,64 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TF* lds = reinterpret_cast<TF*>(lds_raw);
     FPair<TF>* ldsp = reinterpret_cast<FPair<TF>*>(lds_raw + lds_pair_offset<TF>(a.hot_x));
+    // Tiles are dealt to blocks interleaved (block b: tiles b, b + B, b + 2B,
+    // ..., B blocks), so every block samples the whole layout and the blocks'
+    // shares cost about the same; inside a block each wave takes the block's
+    // next tile from a counter in LDS when it finishes one, so the waves of
+    // a block finish within about one tile of each other.  A block runs its
+    // tiles in ascending order (all blocks then stream one window of the
+    // layout), except the costly tail of a split layout (second parts, whose
+    // gathers go to global memory): those tiles are spread evenly over the
+    // block's positions, so the pass does not end on them.  (Dealt
+    // statically, one wave in a block could finish tens of microseconds
+    // after another.)  Which wave runs a tile changes nothing in its results.
+    __shared__ uint32_t s_next;
+    constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
+    if (threadIdx.x == 0) s_next = NW;  // positions 0 .. NW-1 go to the waves in order
     stage_lds<SWEEP_BLOCK>(lds, static_cast<const TF*>(a.x), a.hot_x);
     stage_lds<SWEEP_BLOCK>(ldsp, static_cast<const FPair<TF>*>(a.xp), a.hot_p);
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
-    constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
-    const uint32_t stride = gridDim.x * NW;
-    // wave-uniform tile index: the per-tile descriptors become scalar loads
-    const uint32_t first = __builtin_amdgcn_readfirstlane(blockIdx.x * NW + (threadIdx.x >> 6));
+    const uint32_t B = gridDim.x, b = blockIdx.x;
+    // The block's tiles b + j B, j < cnt; those with j >= jc form the costly
+    // tail (nc of them).  Position p < cnt is a tail position when
+    // floor((p + 1) nc / cnt) > floor(p nc / cnt), and takes tail tile
+    // jc + floor(p nc / cnt); any other position takes head tile
+    // p - floor(p nc / cnt).  A bijection onto j < cnt, each part ascending.
+    const uint32_t cnt = b < a.ntiles ? (a.ntiles - 1 - b) / B + 1 : 0u;
+    const uint32_t c0 = a.spread_tile0;
+    const uint32_t jc = c0 >= a.ntiles ? cnt : c0 <= b ? 0u : min(cnt, (uint32_t)(((uint64_t)c0 - b + B - 1) / B));
+    const uint32_t nc = cnt - jc;
+    auto tile_at = [&](uint32_t p) {
+        uint32_t k0, k1;
+        if (cnt < 0x10000u) {
+            k0 = p * nc / cnt;
+            k1 = (p + 1) * nc / cnt;
+        } else {
+            k0 = (uint32_t)((uint64_t)p * nc / cnt);
+            k1 = (uint32_t)((uint64_t)(p + 1) * nc / cnt);
+        }
+        return b + (k1 > k0 ? jc + k0 : p - k0) * B;
+    };
+    // wave-uniform position: the per-tile descriptors become scalar loads
+    uint32_t pos = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Sweeps: each tile's tile_gmax entry is loaded while the previous tile
     // runs (unconditionally, at a clamped tile index: a load on only one
     // of two merged paths would make the compiler drain every outstanding
     // load at the end of each tile).
-    uint32_t gmax = SWEEP && first < a.ntiles ? ld_const(a.tile_gmax, first) : 0xffffffffu;
-    for (uint32_t tile = first; tile < a.ntiles; tile += stride) {
-        const uint32_t next = tile + stride < a.ntiles ? tile + stride : tile;
+    uint32_t gmax = SWEEP && pos < cnt ? ld_const(a.tile_gmax, tile_at(pos)) : 0xffffffffu;
+    while (pos < cnt) {
+        const uint32_t tile = tile_at(pos);
+        // the wave's next position, taken before this tile runs; positions
+        // only grow, so every wave leaves the loop
+        uint32_t pn = 0;
+        if (lane == 0) pn = atomicAdd(&s_next, 1u);
+        pn = __builtin_amdgcn_readfirstlane(pn);
+        const uint32_t next = pn < cnt ? tile_at(pn) : tile;
         const uint32_t gnext = SWEEP ? ld_const(a.tile_gmax, next) : 0xffffffffu;
         tile_body<IdxT, SIGMA, TF, TA, SUB, ADD, SWEEP, XCUR>(a, lds, ldsp, tile, lane, gmax);
         gmax = gnext;
+        pos = pn;
     }
 }
 

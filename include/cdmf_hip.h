@@ -213,8 +213,7 @@ int32_t ccd_set_plan(ccd_ctx* ctx, const ccd_plan_params* pp);
  * After ccd_batch_run every context holds the result of its own ccd_run
  * with the tiled sweeps on the plain (unsplit) tile layout, i.e. with
  * ccd_plan_params.split_rows = 0 (bit-identical, same sigma; a ccd_run with
- * split_rows = 1 rounds differently wherever a gathered vector exceeds the
- * LDS prefix): read it with ccd_get_factors / ccd_get_residual / ccd_rmse,
+ * split rows rounds differently wherever it splits a row): read it with ccd_get_factors / ccd_get_residual / ccd_rmse,
  * continue it with another batch run or a ccd_run.
  */
 typedef struct ccd_batch ccd_batch;
