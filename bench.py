@@ -189,12 +189,12 @@ def main():
         s.set_factors(W0, H0)
         t = time.perf_counter()
         s.run(k, lam, args.warmup, inner, variant=args.variant, profile=True)
+        all_stats = s.kernel_stats()
+        timed_kinds = (max(sweep_kinds, key=lambda kk: all_stats[kk]["total_ms"]),)
         if args.graph:  # capture the replayed outer iteration before the clock starts
             s.set_factors(W0, H0)
             s.run(k, lam, 2, inner, variant=args.variant, use_graph=True)
         log("warmup %d outer iteration(s) in %.2fs" % (args.warmup, time.perf_counter() - t))
-        all_stats = s.kernel_stats()
-        timed_kinds = (max(sweep_kinds, key=lambda kk: all_stats[kk]["total_ms"]),)
 
     s.set_factors(W0, H0)
     t0 = time.perf_counter()

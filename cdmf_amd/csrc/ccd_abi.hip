@@ -712,17 +712,29 @@ int32_t ccd_upload_test(ccd_ctx* c, uint64_t nnz, const uint32_t* row, const uin
     HIP_TRY(c, dalloc(&c->tcol, nnz));
     HIP_TRY(c, dalloc(&c->tval, nnz));
     if (nnz) {
-        // internal numbering of the test entries (order kept)
+        // Internal numbering of the test entries, sorted by (row, column):
+        // neighbouring threads of the RMSE kernel then read neighbouring
+        // entries of each rank-major factor row, one or two cache lines a
+        // load instead of one per lane.  Only the order of the FP64 sum of
+        // squared errors changes.
         std::vector<uint32_t> rpos((size_t)c->m), cpos((size_t)c->n), r2(nnz), c2(nnz);
+        std::vector<float> v2(nnz);
+        std::vector<uint64_t> key(nnz);
         for (int64_t i = 0; i < c->m; ++i) rpos[c->csr.perm[i]] = (uint32_t)i;
         for (int64_t j = 0; j < c->n; ++j) cpos[c->csc.perm[j]] = (uint32_t)j;
-        for (uint64_t e = 0; e < nnz; ++e) {
-            r2[e] = rpos[row[e]];
-            c2[e] = cpos[col[e]];
+        for (uint64_t e = 0; e < nnz; ++e) key[e] = (uint64_t)rpos[row[e]] << 32 | cpos[col[e]];
+        std::vector<uint64_t> order(nnz);
+        for (uint64_t e = 0; e < nnz; ++e) order[e] = e;
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) { return key[x] < key[y]; });
+        for (uint64_t q = 0; q < nnz; ++q) {
+            const uint64_t e = order[q];
+            r2[q] = (uint32_t)(key[e] >> 32);
+            c2[q] = (uint32_t)key[e];
+            v2[q] = val[e];
         }
         HIP_TRY(c, ccd::copy_sync(c->stream, c->trow, r2.data(), 4 * nnz, hipMemcpyHostToDevice));
         HIP_TRY(c, ccd::copy_sync(c->stream, c->tcol, c2.data(), 4 * nnz, hipMemcpyHostToDevice));
-        HIP_TRY(c, ccd::copy_sync(c->stream, c->tval, val, 4 * nnz, hipMemcpyHostToDevice));
+        HIP_TRY(c, ccd::copy_sync(c->stream, c->tval, v2.data(), 4 * nnz, hipMemcpyHostToDevice));
     }
     return CCD_OK;
 }
