@@ -407,7 +407,18 @@ int build_tiled(ccd_ctx* c, DevLayout& L, int sigma, uint32_t split_at) {
     T.nspan = (uint32_t)H.spans.size();
     T.nshort = H.nshort;
     int rc;
-    if ((rc = upload_vec(c, &T.flags, H.flags)) || (rc = upload_vec(c, &T.row0, H.row0)) ||
+    if (sigma <= 16) {  // the row-start bits of a lane fit 16 bits: half the stream
+        std::vector<uint16_t> f16(H.flags.begin(), H.flags.end());
+        uint16_t* d = nullptr;
+        rc = upload_vec(c, &d, f16);
+        T.flags = d;
+    } else {
+        uint32_t* d = nullptr;
+        rc = upload_vec(c, &d, H.flags);
+        T.flags = d;
+    }
+    if (rc) return rc;
+    if ((rc = upload_vec(c, &T.row0, H.row0)) ||
         (rc = upload_vec(c, &T.segoff, H.segoff)) || (rc = upload_vec(c, &T.segrows, H.segrows)) ||
         (rc = upload_vec(c, &T.open, H.open)) || (rc = upload_vec(c, &T.spans, H.spans)))
         return rc;

@@ -20,7 +20,7 @@ struct TiledLayout {
     uint32_t ntiles = 0;
     void* tidx = nullptr;
     float* tval = nullptr;
-    uint32_t* flags = nullptr;
+    void* flags = nullptr;  // per lane: u16 words for sigma <= 16, u32 above
     uint32_t* row0 = nullptr;
     uint32_t* segoff = nullptr;
     uint32_t* segrows = nullptr;

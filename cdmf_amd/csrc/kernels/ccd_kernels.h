@@ -293,6 +293,7 @@ hipError_t launch_count_below(const uint32_t* ptr, const void* idx, bool idx16, 
 // (i/4)*256 + 4*l + i%4) so a 16-byte load of every lane reads 1 KiB of
 // consecutive values.
 //   lane_flags  bit i: element i of the lane is the first element of a row
+//               (16-bit words for sigma <= 16, 32-bit for larger tiles)
 //   tile_row0   row of the tile's first element
 //   tile_segoff 0xffffffff when the tile's rows are consecutive, else offset
 //               into segrows (row of every segment of the tile)
@@ -315,7 +316,7 @@ struct TiledArgs {
     const void* xa;
     const void* ya;
     const void* xp;
-    const uint32_t* lane_flags;
+    const void* lane_flags;
     const uint32_t* tile_row0;
     const uint32_t* tile_segoff;
     const uint32_t* segrows;
@@ -418,7 +419,7 @@ struct BatchLayout {  // one tiled layout of one problem (see TiledArgs)
     const uint32_t* ptr;
     const void* tidx;
     float* tval;
-    const uint32_t* lane_flags;
+    const void* lane_flags;
     const uint32_t* tile_row0;
     const uint32_t* tile_segoff;
     const uint32_t* segrows;

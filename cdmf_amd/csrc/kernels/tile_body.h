@@ -117,7 +117,8 @@ __device__ __forceinline__ void load_tile(const TiledArgs& a, const uint32_t til
     constexpr uint32_t TS = WAVE * SIGMA;
     const IE* __restrict__ tidx = static_cast<const IE*>(a.tidx) + (size_t)tile * TS;
     const float* __restrict__ tval = a.tval + (size_t)tile * TS;
-    flags = a.lane_flags[(size_t)tile * WAVE + lane];
+    if constexpr (SIGMA <= 16) flags = static_cast<const uint16_t*>(a.lane_flags)[(size_t)tile * WAVE + lane];
+    else flags = static_cast<const uint32_t*>(a.lane_flags)[(size_t)tile * WAVE + lane];
     // ORDERED: keep this load ahead of the streams (the row lookups wait for
     // it, and loads complete in issue order)
     if constexpr (ORDERED) __builtin_amdgcn_sched_barrier(0);
