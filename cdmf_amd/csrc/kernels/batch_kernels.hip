@@ -125,27 +125,54 @@ struct Staged {
     uint32_t hot_x, hot_p;
 };
 
-// The tiles of one pass taken by wave w of nw.
+// Who takes which tile of a pass.  Wave w of the team's nw takes tiles w,
+// w + nw, ...; or, for a sweep over a layout with at least 16 tiles per
+// wave (dyn), block r of the team's nb owns tiles r, r + nb, ... and its
+// waves take the next one from the block's counter in LDS (*pos, set to
+// WPB by the caller: positions 0 .. WPB-1 go to the waves in order) when
+// they finish one, so they finish within about a tile of each other (the
+// largest problems bound the batch).  Which wave runs a tile changes
+// nothing in its results.
+struct Deal {
+    uint32_t w, nw;  // static
+    uint32_t r, nb;  // dynamic
+    uint32_t* pos;   // LDS
+    bool dyn;
+};
+
+template <typename IdxT, int SIGMA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
+__device__ __forceinline__ void tile_loop(const TiledArgs& a, const float* lds, const FPair<float>* ldsp,
+                                          const Deal& d) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (!SWEEP || !d.dyn) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(d.w);  // wave-uniform: scalar descriptor loads
+        for (uint32_t t = w; t < a.ntiles; t += d.nw)
+            tiled::tile_body<IdxT, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
+        return;
+    }
+    const uint32_t cnt = d.r < a.ntiles ? (a.ntiles - 1 - d.r) / d.nb + 1 : 0u;
+    uint32_t p = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    while (p < cnt) {
+        // the wave's next position, taken before this tile runs; positions
+        // only grow, so every wave leaves the loop
+        uint32_t pn = 0;
+        if (lane == 0) pn = atomicAdd(d.pos, 1u);
+        pn = __builtin_amdgcn_readfirstlane(pn);
+        tiled::tile_body<IdxT, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, d.r + p * d.nb,
+                                                                                  lane);
+        p = pn;
+    }
+}
+
 template <int SIGMA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
-__device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, const Staged& st, uint32_t w, uint32_t nw) {
+__device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, const Staged& st, const Deal& d) {
     const BatchLayout& L = *Lp;
     TiledArgs a = layout_args(L, v);
     a.hot_x = st.hot_x;
     a.hot_p = st.hot_p;
-    const float* lds = st.lds;
-    const FPair<float>* ldsp = st.ldsp;
-    const int lane = threadIdx.x & (WAVE - 1);
-    w = __builtin_amdgcn_readfirstlane(w);  // wave-uniform: scalar loads of the tile descriptors
-    if (L.ib == 2) {
-        for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<uint16_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
-    } else if (L.ib == 3) {
-        for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<Idx24, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
-    } else {
-        for (uint32_t t = w; t < L.ntiles; t += nw)
-            tiled::tile_body<uint32_t, SIGMA, float, float, SUB, ADD, SWEEP, XCUR, false>(a, lds, ldsp, t, lane);
-    }
+    if (L.ib == 2) tile_loop<uint16_t, SIGMA, SUB, ADD, SWEEP, XCUR>(a, st.lds, st.ldsp, d);
+    else if (L.ib == 3) tile_loop<Idx24, SIGMA, SUB, ADD, SWEEP, XCUR>(a, st.lds, st.ldsp, d);
+    else tile_loop<uint32_t, SIGMA, SUB, ADD, SWEEP, XCUR>(a, st.lds, st.ldsp, d);
 }
 
 // Modes used by the loop: plain sweep, fused sweeps (subtract, or subtract
@@ -156,16 +183,16 @@ __device__ __forceinline__ void tiles_t(const BatchLayout* Lp, PassVecs v, const
 // here because it makes the compiler spill the whole kernel.)
 template <int SIGMA>
 __device__ __forceinline__ void tiles(const BatchLayout* L, const PassVecs& v, const Staged& st, bool sub, bool add,
-                                      bool sweep, uint32_t w, uint32_t nw) {
+                                      bool sweep, const Deal& d) {
     const int code = (sub ? 4 : 0) | (add ? 2 : 0) | (sweep ? 1 : 0) | (sweep && (sub || add) && v.xcur ? 8 : 0);
     switch (code) {
-        case 1: tiles_t<SIGMA, false, false, true, false>(L, v, st, w, nw); break;
-        case 2: tiles_t<SIGMA, false, true, false, false>(L, v, st, w, nw); break;
-        case 4: tiles_t<SIGMA, true, false, false, false>(L, v, st, w, nw); break;
-        case 5: tiles_t<SIGMA, true, false, true, false>(L, v, st, w, nw); break;
-        case 7: tiles_t<SIGMA, true, true, true, false>(L, v, st, w, nw); break;
-        case 8 + 5: tiles_t<SIGMA, true, false, true, true>(L, v, st, w, nw); break;
-        case 8 + 7: tiles_t<SIGMA, true, true, true, true>(L, v, st, w, nw); break;
+        case 1: tiles_t<SIGMA, false, false, true, false>(L, v, st, d); break;
+        case 2: tiles_t<SIGMA, false, true, false, false>(L, v, st, d); break;
+        case 4: tiles_t<SIGMA, true, false, false, false>(L, v, st, d); break;
+        case 5: tiles_t<SIGMA, true, false, true, false>(L, v, st, d); break;
+        case 7: tiles_t<SIGMA, true, true, true, false>(L, v, st, d); break;
+        case 8 + 5: tiles_t<SIGMA, true, false, true, true>(L, v, st, d); break;
+        case 8 + 7: tiles_t<SIGMA, true, true, true, true>(L, v, st, d); break;
         default: break;  // not produced by the loop below
     }
 }
@@ -228,12 +255,18 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
 
     const Staged none{nullptr, nullptr, 0u, 0u};
 
+    __shared__ uint32_t s_pos;  // the block's next tile position in a dealt sweep
+    const Deal fixed{w, nw, 0u, 0u, nullptr, false};
+
     // one sweep (or fused) pass: LDS staging, tiles, barrier, rows spanning
     // tiles, barrier
     auto pass = [&](const BatchLayout& L, const PassVecs& pv, bool sub, bool add) -> bool {
         __syncthreads();  // every wave of the block is done with the LDS of the last pass
+        const bool dyn = L.ntiles >= 16u * nw;
+        if (dyn && threadIdx.x == 0) s_pos = WPB;
         const Staged st = stage_pass(L, pv, sub || add, batch_lds, ba.lds_floats);
-        tiles<SIGMA>(&L, pv, st, sub, add, true, w, nw);
+        if (dyn) __syncthreads();  // s_pos seen by every wave
+        tiles<SIGMA>(&L, pv, st, sub, add, true, Deal{w, nw, rank, P.nblocks, &s_pos, dyn});
         if (!team_sync(tm)) return false;
         if (L.nspan) {
             const SpanArgs sa{L.spans, L.head, L.tail, pv.y, L.nspan, L.nshort, lambda};
@@ -268,10 +301,12 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
             const bool add = oiter > 1 || P.continued;
             if (add && !pending) {  // rank 0 of a continued run: add back, then plain sweeps
                 tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, nullptr, nullptr, St, Ft, 0u, lambda}, none, false, true,
-                             false, w, nw);
+                             false, fixed);
                 tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, nullptr, nullptr, Ft, St, 0u, lambda}, none, false, true,
-                             false, w, nw);
-                // each wave sweeps the tiles it just updated: no barrier needed
+                             false, fixed);
+                // a dealt sweep may give a tile to another wave than the
+                // add-back did
+                if (!team_sync(tm)) return;
                 if (!pass(P.F, plain(St, P.vF), false, false)) return;
                 if (!pass(P.S, plain(P.vF, P.vS), false, false)) return;
             } else {
@@ -297,9 +332,9 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
         const float* Sp = P.XS + (size_t)prev * dS;
         const float* Fp = P.XF + (size_t)prev * dF;
         tiles<SIGMA>(&P.F, PassVecs{nullptr, nullptr, Sp, Fp, nullptr, nullptr, 0u, lambda}, none, true, false, false,
-                     w, nw);
+                     fixed);
         tiles<SIGMA>(&P.S, PassVecs{nullptr, nullptr, Fp, Sp, nullptr, nullptr, 0u, lambda}, none, true, false, false,
-                     w, nw);
+                     fixed);
     }
 }
 
