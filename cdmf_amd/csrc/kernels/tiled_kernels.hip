@@ -74,13 +74,13 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     const uint32_t c0 = a.spread_tile0;
     const uint32_t jc = c0 >= a.ntiles ? cnt : c0 <= b ? 0u : min(cnt, (uint32_t)(((uint64_t)c0 - b + B - 1) / B));
     const uint32_t nc = cnt - jc;
-    // Fused passes over a layout without a costly tail: positions from
-    // both ends in turn (first, last, second, ...), so the short rows at
-    // the layout's end, which cost a fused pass a little more per tile, are
-    // spread over the pass as well.
-    const bool zigzag = (SUB || ADD) && jc == cnt;
+    // Fused passes take the positions from both ends in turn (first, last,
+    // second, ...) before the map above, so the short rows at a layout's
+    // end, which cost a fused pass a little more per tile, are spread over
+    // the pass as well.
+    const bool zigzag = SUB || ADD;
     auto tile_at = [&](uint32_t p) {
-        if (zigzag) return b + ((p & 1u) ? cnt - 1u - (p >> 1) : (p >> 1)) * B;
+        if (zigzag) p = (p & 1u) ? cnt - 1u - (p >> 1) : (p >> 1);
         uint32_t k0, k1;
         if (cnt < 0x10000u) {
             k0 = p * nc / cnt;
