@@ -333,8 +333,11 @@ def bench_batch(args):
     trial_ms = b.tune()
     plan = b.plan()
     tune_times = b.tune_times()
-    log("autotuned plan %s (trial %.2f ms); whole runs with 0, 1, ... problems solved alone: %s ms" % (
-        plan, trial_ms, ["%.1f" % x for x in tune_times]))
+    n_solo = min(3, len(specs) - 1) + 1
+    tune_solo, tune_deal = tune_times[:n_solo], tune_times[n_solo:]
+    log("autotuned plan %s (%.2f ms); whole runs with 0, 1, ... problems solved alone: %s ms; with the tile deal "
+        "at 8, 32, never (16 above): %s ms" % (plan, trial_ms, ["%.1f" % x for x in tune_solo],
+                                               ["%.1f" % x for x in tune_deal]))
     reset()
     b.run()  # warm-up
     reset()
@@ -374,7 +377,8 @@ def bench_batch(args):
         "speedup_vs_sequential": seq_s / graph_wall,
         "max_test_rmse_diff_vs_sequential": worst,
         "teams": teams,
-        "tune_whole_run_ms_by_solo": tune_times,
+        "tune_whole_run_ms_by_solo": tune_solo,
+        "tune_whole_run_ms_by_deal": dict(zip(("8", "32", "never"), tune_deal)),
         "problems": [dict(m=bp.m, n=bp.n, nnz=nnz, k=bp.solve.k, updates=bp.solve.outer * bp.solve.k * nnz,
                           sequential_s=t) for bp, (_, _, nnz), t in zip(specs, inits, per_problem)],
     }

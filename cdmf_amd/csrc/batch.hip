@@ -24,7 +24,7 @@ struct ccd_batch {
     // problems of the persistent launch (indices into items); the others
     // of a run are solved alone first (ccd_batch_plan.solo)
     std::vector<size_t> active;
-    ccd_batch_plan plan{16, 0, 1.0f, 0};
+    ccd_batch_plan plan{16, 0, 1.0f, 0, 0};
     int ncu = 0;
     int lds_cu = 0, lds_block = 0;  // LDS bytes of a CU and of a block
     uint32_t lds_floats = 0;        // dynamic LDS of a block of the last grid_blocks, in 4-byte entries
@@ -42,7 +42,9 @@ struct ccd_batch {
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     double last_seconds = 0.0;
     std::vector<uint32_t> team;  // blocks per problem of the last launch
-    std::vector<double> tune_solo_ms;  // the last tune's whole-run times, solo = 0, 1, ...
+    // the last tune's whole-run times: solo = 0, 1, ..., then the deal
+    // candidates of tune_deals (with the best solo count)
+    std::vector<double> tune_solo_ms;
 };
 
 namespace {
@@ -186,6 +188,13 @@ Split split_grid(const ccd_batch* b, uint32_t blocks, float expo, int sigma) {
     return sp;
 }
 
+// Tiles per wave from which a team's sweep deals its tiles (plan.deal: 0 is
+// the default).
+constexpr uint32_t DEAL_DEFAULT = 16;
+uint32_t deal_min(uint32_t deal) { return deal ? deal : DEAL_DEFAULT; }
+// deal values the tuner tries after the default
+constexpr uint32_t tune_deals[] = {8u, 32u, 0xffffffffu};
+
 // Upload the launch description (trial: the autotuner's short run).
 int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool trial, ccd::BatchArgs* args) {
     const std::vector<uint32_t>& nb = sp.nb;
@@ -249,6 +258,7 @@ int describe(ccd_batch* b, int sigma, uint32_t blocks, const Split& sp, bool tri
     args->wait_ticks = 100000000ull * 4;  // 4 s of the 100 MHz wall clock
     args->trial = trial ? 1u : 0u;
     args->lds_floats = b->lds_floats;
+    args->deal_min = deal_min(b->plan.deal);
     return CCD_OK;
 }
 
@@ -519,7 +529,7 @@ int32_t ccd_batch_tune(ccd_batch* b, double* best_ms) {
             if (int rc = timed_trial(b, (int)sigma, blocks, expo, &ms)) return rc;
             if (ms < bms) {
                 bms = ms;
-                best = ccd_batch_plan{sigma, 0u, expo, 0u};
+                best = ccd_batch_plan{sigma, 0u, expo, 0u, 0u};
             }
         }
     }
@@ -554,9 +564,30 @@ int32_t ccd_batch_tune(ccd_batch* b, double* best_ms) {
                 best_solo = solo;
             }
         }
+        // Then the tile deal of the sweeps (the default, 16 tiles per wave,
+        // was the runs above): more or fewer teams' sweeps dealt.
+        best.solo = best_solo;
+        uint32_t best_deal = 0;
+        for (uint32_t deal : tune_deals) {
+            if (rc) break;
+            b->plan = best;
+            b->plan.deal = deal;
+            double secs = 0.0;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                drop_graph(b);
+                rc = ccd_batch_run(b, 0);
+                if (!rc) secs = b->last_seconds;
+                if (!rc) rc = restore_snapshot(b, sn);
+            }
+            if (!rc) b->tune_solo_ms.push_back(1e3 * secs);
+            if (!rc && secs < best_s) {
+                best_s = secs;
+                best_deal = deal;
+            }
+        }
         free_snapshot(sn);
         if (rc) return rc;
-        best.solo = best_solo;
+        best.deal = best_deal;
         bms = 1e3 * best_s;
     }
     b->plan = best;
@@ -595,7 +626,7 @@ int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph) {
     // it was captured with: a context's generation moves whenever it
     // reallocates any of them (upload, a new k, a rebuilt tile layout), and
     // the addresses the capture bakes in are keyed as well
-    std::vector<uint64_t> key{(uint64_t)sigma, blocks, (uint64_t)b->active.size()};
+    std::vector<uint64_t> key{(uint64_t)sigma, blocks, (uint64_t)b->active.size(), deal_min(b->plan.deal)};
     for (size_t i : b->active) {
         const ccd_batch::Item& it = b->items[i];
         const ccd_ctx* c = it.ctx;

@@ -126,8 +126,8 @@ struct Staged {
 };
 
 // Who takes which tile of a pass.  Wave w of the team's nw takes tiles w,
-// w + nw, ...; or, for a sweep over a layout with at least 16 tiles per
-// wave (dyn), block r of the team's nb owns tiles r, r + nb, ... and its
+// w + nw, ...; or, for a sweep over a layout with at least ba.deal_min
+// tiles per wave (dyn), block r of the team's nb owns tiles r, r + nb, ... and its
 // waves take the next one from the block's counter in LDS (*pos, set to
 // WPB by the caller: positions 0 .. WPB-1 go to the waves in order) when
 // they finish one, so they finish within about a tile of each other (the
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(BATCH_BLOCK) void batch_kernel(BatchArgs ba) {
     // tiles, barrier
     auto pass = [&](const BatchLayout& L, const PassVecs& pv, bool sub, bool add) -> bool {
         __syncthreads();  // every wave of the block is done with the LDS of the last pass
-        const bool dyn = L.ntiles >= 16u * nw;
+        const bool dyn = (uint64_t)L.ntiles >= (uint64_t)ba.deal_min * nw;
         if (dyn && threadIdx.x == 0) s_pos = WPB;
         const Staged st = stage_pass(L, pv, sub || add, batch_lds, ba.lds_floats);
         if (dyn) __syncthreads();  // s_pos seen by every wave

@@ -462,6 +462,7 @@ struct BatchArgs {
     uint64_t wait_ticks;            // bound of one barrier wait in wall_clock64() ticks
     uint32_t trial;                 // autotuner trial: trial_reps plain sweep pairs per problem, nothing else
     uint32_t lds_floats;            // dynamic LDS of a block, in 4-byte entries (staged hot prefixes)
+    uint32_t deal_min;              // dealt sweeps: layouts with at least this many tiles per wave of the team
 };
 
 // sigma 8 or 16; dynamic LDS of a.lds_floats entries per block

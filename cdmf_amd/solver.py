@@ -45,12 +45,16 @@ class CPlan(C.Structure):
 
 
 class CBatchPlan(C.Structure):
-    _fields_ = [("sigma", C.c_uint32), ("blocks", C.c_uint32), ("work_exponent", C.c_float), ("solo", C.c_uint32)]
+    _fields_ = [("sigma", C.c_uint32), ("blocks", C.c_uint32), ("work_exponent", C.c_float), ("solo", C.c_uint32),
+                ("deal", C.c_uint32)]
 
 
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double)
 
 _hip = None
+
+
+ABI_VERSION = 6  # CCD_ABI_VERSION of include/cdmf_hip.h this binding is written against
 
 
 def hip_lib() -> C.CDLL:
@@ -95,6 +99,10 @@ def hip_lib() -> C.CDLL:
             f = getattr(lib, name)
             f.argtypes = args
             f.restype = res
+        got = lib.ccd_abi_version()
+        if got != ABI_VERSION:
+            raise RuntimeError("libcdmf_hip.so has ABI version %d, this binding expects %d (rebuild the library)"
+                               % (got, ABI_VERSION))
         _hip = lib
     return _hip
 
@@ -347,15 +355,19 @@ class Batch:
         self.solvers.append(solver)
         return idx
 
-    def set_plan(self, sigma: int = 16, blocks: int = 0, work_exponent: float = 1.0, solo: int = 0) -> None:
+    def set_plan(self, sigma: int = 16, blocks: int = 0, work_exponent: float = 1.0, solo: int = 0,
+                 deal: int = 0) -> None:
         """solo: the largest problems solved alone on the whole GPU first
-        (ccd_batch_plan.solo)."""
-        self._check(self._lib.ccd_batch_set_plan(self._b, C.byref(CBatchPlan(sigma, blocks, work_exponent, solo))))
+        (ccd_batch_plan.solo); deal: tiles per wave from which a team's
+        sweep deals its tiles through an LDS counter (0: the default, 16;
+        0xffffffff: never)."""
+        self._check(self._lib.ccd_batch_set_plan(self._b, C.byref(CBatchPlan(sigma, blocks, work_exponent, solo,
+                                                                              deal))))
 
     def plan(self) -> dict:
         pp = CBatchPlan()
         self._check(self._lib.ccd_batch_get_plan(self._b, C.byref(pp)))
-        return dict(sigma=pp.sigma, blocks=pp.blocks, work_exponent=pp.work_exponent, solo=pp.solo)
+        return dict(sigma=pp.sigma, blocks=pp.blocks, work_exponent=pp.work_exponent, solo=pp.solo, deal=pp.deal)
 
     def tune(self) -> float:
         ms = C.c_double()
@@ -371,7 +383,8 @@ class Batch:
         return s.value
 
     def tune_times(self) -> list:
-        """Whole-run times (ms) of the last tune for solo = 0, 1, ... (ccd_batch_tune_times)."""
+        """Whole-run times (ms) of the last tune: solo = 0, 1, ..., min(3,
+        problems - 1), then deal = 8, 32, never (ccd_batch_tune_times)."""
         buf = np.zeros(16)
         n = self._check(self._lib.ccd_batch_tune_times(self._b, N.ptr(buf, C.c_double), buf.size))
         return [float(x) for x in buf[:min(n, buf.size)]]

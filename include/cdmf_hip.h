@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CCD_ABI_VERSION 5
+#define CCD_ABI_VERSION 6
 
 /* Status codes: 0 = success, negative = error (message: ccd_last_error). */
 #define CCD_OK 0
@@ -228,6 +228,11 @@ typedef struct ccd_batch_plan {
                              launch takes the rest (default 0, at most 64).  A team of the launch is
                              confined to one XCD, so a problem with more than an eighth of the work bounds
                              the launch. */
+    uint32_t deal;        /* a team's sweep over a layout with at least `deal` tiles per wave of the team
+                             deals them through a counter in LDS (blocks own interleaved tiles, waves take
+                             the next one when they finish one); smaller layouts are split statically
+                             among the waves.  0 = the default, 16; 0xffffffff = never.  The results do
+                             not depend on it. */
 } ccd_batch_plan;
 
 int32_t ccd_batch_create(int32_t device, ccd_batch** out);
@@ -240,10 +245,11 @@ int32_t ccd_batch_get_plan(ccd_batch* b, ccd_batch_plan* out);
 /* Autotuner: times short trial launches (about 1/64 of each problem's
  * sweeps, no state change) for every sigma and work exponent and keeps the
  * fastest; then, when no problem continues an earlier run, times whole runs
- * with 0 to 3 problems solved alone (solo), putting every problem's factors
- * and residual back to their start after each, and keeps the fastest.
- * *best_ms = the kept plan's trial time, or its whole-run time when solo
- * counts were timed. */
+ * with 0 to 3 problems solved alone (solo), then with the tile deal at 8, 32
+ * and never (deal; the solo runs used the default, 16), putting every
+ * problem's factors and residual back to their start after each, and keeps
+ * the fastest.  *best_ms = the kept plan's trial time, or its whole-run time
+ * when whole runs were timed. */
 int32_t ccd_batch_tune(ccd_batch* b, double* best_ms);
 /* One run of every problem: set-up fills and retiling, the persistent
  * launch, write-back.  use_graph: capture that sequence in a HIP graph on
@@ -254,8 +260,9 @@ int32_t ccd_batch_run(ccd_batch* b, int32_t use_graph);
 /* time of the last run: host time of the problems solved alone plus the
  * device time of the launch sequence (events around it) */
 int32_t ccd_batch_time(ccd_batch* b, double* seconds);
-/* Whole-run times (ms) the last ccd_batch_tune measured for solo = 0, 1, ...
- * (none when it timed trials only); returns how many there are. */
+/* Whole-run times (ms) the last ccd_batch_tune measured: solo = 0, 1, ...,
+ * min(3, problems - 1), then deal = 8, 32, 0xffffffff with the best solo
+ * count (none when it timed trials only); returns how many there are. */
 int32_t ccd_batch_tune_times(ccd_batch* b, double* ms, int32_t max_count);
 /* blocks given to each problem by the last run (0: solved alone); returns
  * the problem count */

@@ -38,7 +38,7 @@ def test_no_device_fails_cleanly():
         pytest.skip("a GPU is visible")
     with pytest.raises(solver.CCDError):
         solver.Solver(0)
-    assert solver.hip_lib().ccd_abi_version() == 5
+    assert solver.hip_lib().ccd_abi_version() == 6
 
 
 def test_params_default_are_the_reference_defaults():

@@ -114,6 +114,26 @@ def test_batch_bitwise_equals_single_runs(sigma):
     b.close()
 
 
+@pytest.mark.parametrize("deal", [1, 0xffffffff])
+def test_batch_tile_deal_bitwise(deal):
+    """ccd_batch_plan.deal only changes which wave runs a tile: with every
+    sweep dealt through the blocks' LDS counters (1 tile per wave) and with
+    none dealt, every problem's W, H and residual equal the single-problem
+    runs bit for bit (a continued run included: its first rank's add-back
+    and first sweep meet at a team barrier when the sweep is dealt)."""
+    S = _solver()
+    cases = _cases()
+    b, solvers = _build(S, cases, 16)
+    b.set_plan(sigma=16, deal=deal)
+    assert b.plan()["deal"] == deal
+    b.run()
+    b.run(use_graph=True)  # continued: two runs
+    got = _batch_results(solvers)
+    for (name, ds, k, lam, outer, inner, order, W0, H0), g in zip(cases, got):
+        _same(g, _single(S, ds, k, lam, outer, inner, order, W0, H0, 16, runs=2), "%s deal %d" % (name, deal))
+    b.close()
+
+
 def test_batch_small_grid_stages_in_lds():
     """16 blocks for 5 problems (two per XCD): every team streams far more
     elements per block than it would stage, so each pass stages the gathered
