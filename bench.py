@@ -79,7 +79,7 @@ def cpu_baseline(ds, W0, k, lam, inner, budget_s=15.0):
     }
 
 
-def survey_bytes(kind, m, n, nnz):
+def reference_format_bytes(kind, m, n, nnz):
     """Algorithmic HBM bytes of one launch in the reference's own format (u32
     indices, f32 values, f32 vectors): the byte model of the path.
     sweep over the CSC copy (v update): 8 nnz + 4(n+1) + 4m + 4n;
@@ -211,7 +211,7 @@ def main():
         all_stats = stats
     avg_ms = d["total_ms"] / max(d["launches"], 1)
     # H-first: the first-swept layout is the CSC copy
-    model_bytes = survey_bytes(dom, ds.m, ds.n, ds.nnz)
+    model_bytes = reference_format_bytes(dom, ds.m, ds.n, ds.nnz)
     achieved = model_bytes / (avg_ms * 1e-3) / 1e9
     achieved_fmt = d["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
     kernels = {kk: dict(avg_us=1e3 * v["total_ms"] / max(v["launches"], 1), launches=v["launches"],
