@@ -33,12 +33,17 @@
 // Built with -ffp-contract=off like the binned kernels.
 #include "tile_body.h"
 
+#include <algorithm>
+
 namespace ccd {
 namespace {
 
 constexpr int WAVE = tiled::WAVE;
 constexpr int BLOCK = 256;
 constexpr int WPB = BLOCK / WAVE;
+// Blocks of a finish launch: 8 per CU of a 256-CU GPU, one resident round
+// of waves at this kernel's occupancy.
+constexpr uint32_t FINISH_GRID = 2048;
 using tiled::tile_body;
 
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
@@ -114,7 +119,8 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
 }
 
 // Blocks [0, ceil(nshort / BLOCK)) finish two-tile spans, one per thread;
-// the rest finish longer spans, one per wave.
+// the rest (at most FINISH_GRID blocks, grid-stride) finish longer spans,
+// one per wave.
 template <typename TF, typename TA>
 __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
     const uint32_t sb = (a.nshort + BLOCK - 1) / BLOCK;
@@ -123,9 +129,9 @@ __global__ __launch_bounds__(BLOCK) void tiled_span_kernel(SpanArgs a) {
         if (i < a.nshort) tiled::span_short<TF, TA>(a, i);
         return;
     }
-    const uint32_t i = a.nshort + (blockIdx.x - sb) * WPB + (threadIdx.x >> 6);
-    if (i >= a.nspan) return;
-    tiled::span_row<TF, TA>(a, i, threadIdx.x & (WAVE - 1));
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint32_t i = a.nshort + (blockIdx.x - sb) * WPB + w; i < a.nspan; i += (gridDim.x - sb) * WPB)
+        tiled::span_row<TF, TA>(a, i, threadIdx.x & (WAVE - 1));
 }
 
 // map (optional): layout position of each element in tiled order
@@ -175,11 +181,13 @@ __global__ __launch_bounds__(BLOCK) void retile_idx24_kernel(const uint32_t* __r
     }
 }
 
-// One wave per row of a split layout (tiled::split_row).
+// One wave per row of a split layout (tiled::split_row), grid-stride: the
+// grid is at most FINISH_GRID blocks, about one resident round of waves.
 template <typename TF, typename TA>
 __global__ __launch_bounds__(BLOCK) void split_finish_kernel(SplitArgs a) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r < a.nreal) tiled::split_row<TF, TA>(a, r, threadIdx.x & (WAVE - 1));
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint32_t r = blockIdx.x * WPB + w; r < a.nreal; r += gridDim.x * WPB)
+        tiled::split_row<TF, TA>(a, r, threadIdx.x & (WAVE - 1));
 }
 
 // Largest index of each tile, one wave per tile (grid-stride over tiles);
@@ -301,7 +309,7 @@ int tiled_blocks_per_cu(int sigma, int prec, int ib, bool sub, bool add, bool sw
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
     if (a.nspan == 0) return hipSuccess;
     if (a.nshort > a.nspan) return hipErrorInvalidValue;
-    const dim3 grid((a.nshort + BLOCK - 1) / BLOCK + (a.nspan - a.nshort + WPB - 1) / WPB);
+    const dim3 grid((a.nshort + BLOCK - 1) / BLOCK + std::min<uint32_t>((a.nspan - a.nshort + WPB - 1) / WPB, FINISH_GRID));
     switch (prec) {
         case PREC_F32: hipLaunchKernelGGL((tiled_span_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
         case PREC_BF16: hipLaunchKernelGGL((tiled_span_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
@@ -313,7 +321,7 @@ hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s) {
 
 hipError_t launch_split_finish(const SplitArgs& a, int prec, hipStream_t s) {
     if (a.nreal == 0) return hipSuccess;
-    const dim3 grid((a.nreal + WPB - 1) / WPB);
+    const dim3 grid(std::min<uint32_t>((a.nreal + WPB - 1) / WPB, FINISH_GRID));
     switch (prec) {
         case PREC_F32: hipLaunchKernelGGL((split_finish_kernel<float, float>), grid, dim3(BLOCK), 0, s, a); break;
         case PREC_BF16: hipLaunchKernelGGL((split_finish_kernel<bf16, float>), grid, dim3(BLOCK), 0, s, a); break;
