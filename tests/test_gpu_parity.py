@@ -19,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.helpers import golden_dataset, load_golden
+from tests.helpers import DEGENERATE, degenerate, golden_dataset, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -134,6 +134,28 @@ def test_small_wfirst_zero_w(oracle_lib):
     ref = oracle_lib.ccd(ds, W0, H0, c.k, c.lam, c.outer, c.inner, order=1)
     assert np.array_equal(ref[2], g["rmse"])
     _compare(*got, *ref, "small w-first")
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("name", DEGENERATE)
+def test_edge_shapes(oracle_lib, name, variant):
+    """One row over three tiles, one column, a 3 x 2 matrix in one padded
+    tile, and a full 40 x 40 block at rank k = 1, each variant: test RMSE
+    within 1e-4 of the oracle per outer iteration, and W, H and the residual
+    held to the FP64 oracle with the usual bar.  Why FP64 for those: the
+    one-row case sums 2,500 same-sign products per row update, and the FP32
+    oracle's sequential sum lands 2.6e-5 off the FP64 answer in W (6.5e-5 in
+    H; its residual, 0.6 % of R, 1.7e-3 off in relative terms) -- exactly the
+    distance of the GPU, whose sums are trees, from the FP32 oracle."""
+    ds = degenerate(name)
+    k = 1 if name == "dense" else 4
+    _say("case edge %s variant %d (%d x %d, %d ratings, k %d)" % (name, variant, ds.m, ds.n, ds.csr_idx.size, k))
+    W0, H0 = oracle_lib.initial_col(k, ds.m), oracle_lib.initial_col(k, ds.n)
+    got = _run_gpu(ds, W0, H0, k, 0.05, 3, 3, variant=variant)
+    ref = oracle_lib.ccd(ds, W0, H0, k, 0.05, 3, 3)
+    np.testing.assert_allclose(got[2], ref[2], rtol=0, atol=RMSE_ATOL)
+    ref64 = oracle_lib.ccd(ds, W0, H0, k, 0.05, 3, 3, dtype="f64")
+    _compare(*got, *ref64, "edge %s v%d (FP64 oracle)" % (name, variant))
 
 
 def test_split_rows_and_groups(oracle_lib):

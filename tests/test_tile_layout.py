@@ -10,8 +10,9 @@ Cases: the 2,000 x 1,000 matrix (both layouts), the same with every 3rd row
 and 5th column emptied (segment tables), a tall matrix with 70,000 rows
 (3-byte CSC indices, split at 40,704 like the library on MI355X), the
 90,000 x 2,500 power-law matrix of the split parity test (split CSC copy),
-and the reference's stress shape b (a dense row and column of 1,024,000
-ratings: both layouts split, 3-byte indices, spans of about 1,000 tiles).
+the reference's stress shape b (a dense row and column of 1,024,000
+ratings: both layouts split, 3-byte indices, spans of about 1,000 tiles),
+and edge shapes (one row, one column, a 3 x 2 matrix, a full 40 x 40 block).
 """
 import os
 import shutil
@@ -20,7 +21,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.helpers import ROOT
+from tests.helpers import DEGENERATE, ROOT, degenerate
 
 SRC = os.path.join(ROOT, "tests", "native", "tile_check.cpp")
 LAYOUT = os.path.join(ROOT, "cdmf_amd", "csrc", "tile_layout.cpp")
@@ -59,6 +60,8 @@ def _layouts(name):
         r, c = key // n, key % n
         v = np.ones(r.size, np.float32)
         ds = datadir.from_triplets(m, n, (r, c, v), (r[:10], c[:10], v[:10]))
+    elif name in DEGENERATE:
+        ds = degenerate(name)
     elif name == "powerlaw":
         ds = D.gen_powerlaw(m=90000, n=2500, nnz_train=450000, nnz_test=20000, max_row_degree=2500, rating_kind=1,
                             seed=21)
@@ -77,7 +80,7 @@ def _write(path, rows, gdim, ptr, idx):
 
 
 @pytest.mark.parametrize("sigma", [8, 16, 32])
-@pytest.mark.parametrize("name", ["small", "holey", "tall", "powerlaw", "stress_b"])
+@pytest.mark.parametrize("name", ["small", "holey", "tall", "powerlaw", "stress_b"] + DEGENERATE)
 def test_tile_tables_decode_to_the_layout(checker, tmp_path, name, sigma):
     for label, rows, gdim, ptr, idx in _layouts(name):
         path = str(tmp_path / ("%s_%s.bin" % (name, label)))
