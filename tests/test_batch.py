@@ -114,6 +114,28 @@ def test_batch_bitwise_equals_single_runs(sigma):
     b.close()
 
 
+def test_batch_edge_shapes():
+    """One row over three tiles, one column, a 3 x 2 matrix in one padded
+    tile and a full 40 x 40 block at k = 1, batched with one problem of the
+    usual kind: bitwise equal to their single-problem runs."""
+    from cdmf_amd import data as D
+    from cdmf_amd.solver import ORDER_H_FIRST
+    from tests.helpers import DEGENERATE, degenerate
+    S = _solver()
+    cases = [_cases()[3]]
+    for name in DEGENERATE:
+        ds = degenerate(name)
+        k = 1 if name == "dense" else 4
+        cases.append((name, ds, k, 0.05, 3, 3, ORDER_H_FIRST, D.initial_col(k, ds.m), np.zeros((k, ds.n), np.float32)))
+    b, solvers = _build(S, cases, 16)
+    b.run()
+    _say("teams %s" % b.teams())
+    for (name, ds, k, lam, outer, inner, order, W0, H0), g in zip(cases, _batch_results(solvers)):
+        _say("compare %s" % name)
+        _same(g, _single(S, ds, k, lam, outer, inner, order, W0, H0, 16), name)
+    b.close()
+
+
 @pytest.mark.parametrize("deal", [1, 0xffffffff])
 def test_batch_tile_deal_bitwise(deal):
     """ccd_batch_plan.deal only changes which wave runs a tile: with every
