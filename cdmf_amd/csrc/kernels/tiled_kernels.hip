@@ -29,7 +29,9 @@
 // take a copy of the body with plain LDS reads.  A layout whose gathered
 // vector exceeds the prefix can hold its rows split in two virtual rows
 // (first the parts gathered from the prefix, then the rest), finished by
-// split_finish_kernel.  No atomics: results are run-to-run deterministic.
+// split_finish_kernel.  No atomics in the arithmetic (the block's LDS
+// counter only decides which wave runs a tile, and a tile's results do not
+// depend on that): results are run-to-run deterministic.
 // Built with -ffp-contract=off like the binned kernels.
 #include "tile_body.h"
 
