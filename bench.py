@@ -174,7 +174,8 @@ def main():
         log("copy probe: 256 MiB in %.1f us (%.0f GB/s read+write)" % (ms * 1e3, 2 * (256 << 20) / (ms * 1e-3) / 1e9))
     t = time.perf_counter()
     s.upload(ds)
-    log("uploaded in %.1fs" % (time.perf_counter() - t))
+    upload_s = time.perf_counter() - t
+    log("uploaded in %.1fs" % upload_s)
 
     # The warmup times every kernel class (event records between kernels add
     # a few microseconds each); the timed run times only the dominant sweep
@@ -262,6 +263,10 @@ def main():
         "roofline": roofline,
         "test_rmse": res.rmse,
         "loop_time_s": s.loop_time(),
+        # ccd_upload from host buffers (degree renumbering, row sorting and
+        # both compressed copies over PCIe): outside `value`, which starts
+        # with the ratings resident in HBM
+        "upload_s": upload_s,
         "kernels_warmup": kernels,
     }
     s.close()
