@@ -644,7 +644,8 @@ int32_t ccd_set_plan(ccd_ctx* c, const ccd_plan_params* pp) {
     if (!c || !pp) return CCD_ERR_INVALID;
     if (pp->elems_per_lane == 0 || pp->piece == 0 || pp->unit_elems == 0 || pp->split_threshold < pp->piece)
         return set_err(c, CCD_ERR_INVALID, "plan: elems_per_lane, piece, unit_elems > 0 and split_threshold >= piece");
-    if (pp->sigma != 8 && pp->sigma != 16) return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8 or 16");
+    if (pp->sigma != 8 && pp->sigma != 16 && pp->sigma != 32)
+        return set_err(c, CCD_ERR_INVALID, "plan: sigma must be 8, 16 or 32");
     if (pp->blocks_per_cu > 2) return set_err(c, CCD_ERR_INVALID, "plan: blocks_per_cu must be 0, 1 or 2");
     if (pp->split_rows > 1) return set_err(c, CCD_ERR_INVALID, "plan: split_rows must be 0 or 1");
     c->plan = *pp;

@@ -381,9 +381,9 @@ hipError_t launch_split_finish(const SplitArgs& a, int prec, hipStream_t s);
 // bytes of lane l of tile t are the sigma bytes at t * 64 * sigma + l * sigma.
 struct Idx24 {};
 
-// sigma 8 or 16 for FP32, 16 for the other precisions (at sigma 32 the
-// kernels spill hundreds of bytes per lane, run slower, and the fused
-// passes' compiled code gave wrong results); ib: bytes per index
+// sigma 8, 16 or 32 for FP32 (32 in 512-thread blocks: at 1024 threads
+// its kernels spilled to scratch and the spilled passes gave wrong
+// results), 16 for the other precisions; ib: bytes per index
 hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, int ib, bool sub, bool add, bool sweep,
                         uint32_t blocks, hipStream_t s, const LaunchEvents* ev = nullptr);
 hipError_t launch_span(const SpanArgs& a, int prec, hipStream_t s);

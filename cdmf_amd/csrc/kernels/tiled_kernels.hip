@@ -48,8 +48,13 @@ constexpr int WPB = BLOCK / WAVE;
 constexpr uint32_t FINISH_GRID = 2048;
 using tiled::tile_body;
 
+// Threads per block of the tiled kernels: 1024 up to sigma 16; 512 for
+// sigma 32, whose lanes hold 32 elements each and need more than the 128
+// registers a 1024-thread block leaves a lane.
+constexpr int tile_block(int sigma) { return sigma > 16 ? 512 : SWEEP_BLOCK; }
+
 template <typename IdxT, int SIGMA, typename TF, typename TA, bool SUB, bool ADD, bool SWEEP, bool XCUR>
-__global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
+__global__ __launch_bounds__(tile_block(SIGMA)) void tiled_kernel(TiledArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TF* lds = reinterpret_cast<TF*>(lds_raw);
     FPair<TF>* ldsp = reinterpret_cast<FPair<TF>*>(lds_raw + lds_pair_offset<TF>(a.hot_x));
@@ -65,10 +70,11 @@ __global__ __launch_bounds__(SWEEP_BLOCK) void tiled_kernel(TiledArgs a) {
     // statically, one wave in a block could finish tens of microseconds
     // after another.)  Which wave runs a tile changes nothing in its results.
     __shared__ uint32_t s_next;
-    constexpr uint32_t NW = SWEEP_BLOCK / WAVE;
+    constexpr int BS = tile_block(SIGMA);
+    constexpr uint32_t NW = BS / WAVE;
     if (threadIdx.x == 0) s_next = NW;  // positions 0 .. NW-1 go to the waves in order
-    stage_lds<SWEEP_BLOCK>(lds, static_cast<const TF*>(a.x), a.hot_x);
-    stage_lds<SWEEP_BLOCK>(ldsp, static_cast<const FPair<TF>*>(a.xp), a.hot_p);
+    stage_lds<BS>(lds, static_cast<const TF*>(a.x), a.hot_x);
+    stage_lds<BS>(ldsp, static_cast<const FPair<TF>*>(a.xp), a.hot_p);
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t B = gridDim.x, b = blockIdx.x;
@@ -262,6 +268,7 @@ TiledFn tiled_fn_i(int sigma, int prec, int code) {
         switch (sigma) {
             case 8: return tiled_fn_s<IdxT, 8, float, float>(code);
             case 16: return tiled_fn_s<IdxT, 16, float, float>(code);
+            case 32: return tiled_fn_s<IdxT, 32, float, float>(code);
             default: return nullptr;
         }
     }
@@ -289,12 +296,13 @@ hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, int ib, bool su
     if (!sub && !add && !sweep) return record_unlaunched(ev, s);
     const TiledFn fn = tiled_fn(sigma, prec, ib, sub, add, sweep, a.x_is_cur != 0);
     if (!fn) return hipErrorInvalidValue;
-    const uint32_t need = (a.ntiles + SWEEP_BLOCK / WAVE - 1) / (SWEEP_BLOCK / WAVE);
+    const int bs = tile_block(sigma);
+    const uint32_t need = (a.ntiles + bs / WAVE - 1) / (bs / WAVE);
     if (need == 0) return record_unlaunched(ev, s);
     if (blocks > need) blocks = need;
     TiledArgs arg = a;
     void* args[] = {&arg};
-    return hipExtLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(SWEEP_BLOCK), args,
+    return hipExtLaunchKernel(reinterpret_cast<const void*>(fn), dim3(blocks), dim3(bs), args,
                               staged_lds_bytes(prec, a.hot_x, a.hot_p), s, ev ? ev->start : nullptr,
                               ev ? ev->stop : nullptr, 0);
 }
@@ -302,8 +310,8 @@ hipError_t launch_tiled(const TiledArgs& a, int sigma, int prec, int ib, bool su
 int tiled_blocks_per_cu(int sigma, int prec, int ib, bool sub, bool add, bool sweep, bool xcur) {
     const TiledFn fn = tiled_fn(sigma, prec, ib, sub, add, sweep, xcur);
     int nb = 0;
-    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), SWEEP_BLOCK, 0) !=
-                   hipSuccess)
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), tile_block(sigma),
+                                                            0) != hipSuccess)
         return 1;
     return nb < 1 ? 1 : nb;
 }

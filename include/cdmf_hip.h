@@ -191,7 +191,8 @@ typedef struct ccd_plan_params {
     uint32_t split_threshold; /* rows longer than this are split across waves (default 8192) */
     uint32_t piece;           /* elements per piece of a split row (default 4096) */
     uint32_t unit_elems;      /* target elements per wave work unit (default 4096) */
-    uint32_t sigma;           /* elements per lane of a tile of the tiled sweeps: 8 or 16 (default 16) */
+    uint32_t sigma;           /* elements per lane of a tile of the tiled sweeps: 8, 16 or 32 (default 16; 32 runs
+                                 512-thread blocks, FP32 only) */
     uint32_t blocks_per_cu;   /* persistent sweep blocks per CU: 0 = as many as registers allow (1 or 2),
                                  1 or 2 = at most that many; the staged LDS prefix is sized to match */
     uint32_t split_rows;      /* tiled sweeps, layouts whose gathered vector exceeds the LDS prefix: 1 = keep

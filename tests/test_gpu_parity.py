@@ -413,19 +413,15 @@ def _holey(ds):
                                  (ds.test_row, ds.test_col, ds.test_val))
 
 
-# Tiles of 64 x 32 elements: refused by ccd_set_plan.  Compiled for gfx950
-# the sigma-32 kernels spill past the 128 VGPRs of a 1024-thread block and
-# gave wrong results on every multi-tile case of these two tests (the
-# 2,000 x 1,000 case: max |dW| 1.02, max |dRMSE| 4.1e-3), while one-tile
-# cases passed.  The host tables and a restatement of the tile algorithm
-# are exact at sigma 32 under the address and undefined-behaviour
-# sanitizers (tests/test_tile_layout.py), so the fault is in the compiled
-# sigma-32 kernels; its cause is not shown, so sigma 32 stays off.
-SIGMA32_SKIP = pytest.mark.skip(reason="sigma 32 is refused: wrong results of the compiled sigma-32 tile kernels on "
-                                       "multi-tile layouts, cause not shown (host tables verified on the CPU)")
-
-
-@pytest.mark.parametrize("sigma", [8, 16, pytest.param(32, marks=SIGMA32_SKIP)])
+# Tiles of 64 x 32 elements run in 512-thread blocks.  Built for
+# 1024-thread blocks (128 registers a lane) the sigma-32 kernels that update
+# the residual and sweep in one pass spilled 39 to 176 registers a lane to
+# scratch memory, and gave wrong results on every multi-tile case of these
+# two tests (the 2,000 x 1,000 case: max |dW| 1.02), while the kernels that
+# did not spill (a plain sweep, an add-only or subtract-only first pass)
+# agreed with the oracle.  With 512 threads a lane has 256 registers and no
+# tile kernel touches scratch.
+@pytest.mark.parametrize("sigma", [8, 16, 32])
 def test_tiled_variant(oracle_lib, sigma):
     from cdmf_amd import data as D
     cases = []
@@ -479,7 +475,7 @@ def _compare_tall(ds, got, ref, label):
     _close_bulk(got[4], ref[4], label + " residual CSC", quantile=0.9999)
 
 
-@pytest.mark.parametrize("sigma", [8, 16, pytest.param(32, marks=SIGMA32_SKIP)])
+@pytest.mark.parametrize("sigma", [8, 16, 32])
 def test_tiled_3byte_indices(oracle_lib, sigma):
     ds = _tall((1 << 16) + 5000, 400, 30000, 12, seed=11)
     k, lam = 4, 0.05
@@ -502,19 +498,20 @@ def test_tiled_32bit_indices(oracle_lib):
 
 
 def test_plan_checks():
-    """ccd_set_plan rejects sigma 32 (no longer offered) and split_rows
+    """ccd_set_plan rejects sigma other than 8, 16 or 32 and split_rows
     other than 0 or 1 with CCD_ERR_INVALID and a message."""
     solver = _solver()
     with solver.Solver(0) as s:
-        for bad in (dict(sigma=32), dict(split_rows=2)):
+        for bad in (dict(sigma=64), dict(sigma=12), dict(split_rows=2)):
             with pytest.raises(solver.CCDError) as e:
                 s.set_plan(**bad)
             assert "plan" in str(e.value)
         s.set_plan(sigma=8, split_rows=0)
 
 
+@pytest.mark.parametrize("sigma", [16, 32])
 @pytest.mark.parametrize("order", [0, 1])
-def test_tiled_split_rows(oracle_lib, order):
+def test_tiled_split_rows(oracle_lib, order, sigma):
     """Rows split at the LDS prefix (plan split_rows, the default): with
     90,000 power-law rows, most of them rated, the CSC copy's rows have parts
     gathered from beyond the 40,704-entry prefix, kept in tiles of their own
@@ -540,9 +537,9 @@ def test_tiled_split_rows(oracle_lib, order):
     W64, H64, _, res64, _ = oracle_lib.ccd(ds, W0, H0, k, lam32, outer, inner, order=order, dtype="f64")
     out = {}
     for split, graph in ((1, False), (1, True), (0, False)):
-        _say("case split_rows=%d graph=%s order %d: m=%d nnz=%d" % (split, graph, order, ds.m, ds.nnz))
+        _say("case split_rows=%d graph=%s order %d sigma %d: m=%d nnz=%d" % (split, graph, order, sigma, ds.m, ds.nnz))
         with solver.Solver(0) as s:
-            s.set_plan(split_rows=split)
+            s.set_plan(split_rows=split, sigma=sigma)
             s.upload(ds)
             s.set_factors(W0, H0)
             res = s.run(k, lam, outer, inner, variant=3, order=order, use_graph=graph)
